@@ -1,0 +1,283 @@
+"""Elemental-generation benchmark on one MI355X.
+
+Workload (default): 32^3 x 64 lattice, Nev = 200, 33 momenta (|n|^2 <= 4),
+13 derivative combinations (num_nabla = 2), Haar-random SU(3) links,
+orthonormal complex64 eigenvectors -- the shape on which the mixed form is
+required to reach >= 5x the FP64 form at <= 1e-6 normwise error.
+
+One step = elementals of every timeslice (calc_all over all Lt timeslices)
+with links and eigenvectors already resident in HBM and the output written
+to HBM.  value = Lt * P * D * Ne^2 / seconds per step of the mixed form; the
+FP64 form is measured on the same inputs and reported beside it, together
+with the mixed form's error against the FP64 form.
+
+Prints one JSON line.  Only one GPU is used; --gpus above 1 is refused.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "elementals/s (t x p x d x Nev^2) mixed vs FP64 on 1 GPU at <=1e-6 rel err"
+PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s dense MFMA: FP64, FP32
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (latt [Lx,Ly,Lz,Lt], Ne, num_nabla, max |n|^2, links)
+    "C": ([32, 32, 32, 64], 200, 2, 4, "haar"),
+    "B": ([16, 16, 16, 128], 100, 2, 4, "smooth"),
+    "small": ([16, 16, 16, 8], 64, 2, 4, "haar"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def blockwise_error(E, Eref):
+    """max over (d,p) of max|dE| / max|Eref| (torch tensors [Lt, D, P, Ne, Ne]), and the mean."""
+    import torch
+
+    nd, npm = E.shape[1], E.shape[2]
+    dmax = torch.zeros((nd, npm), dtype=torch.float64, device=E.device)
+    rmax = torch.zeros_like(dmax)
+    for d in range(nd):  # one derivative at a time bounds the temporary memory
+        diff = (E[:, d] - Eref[:, d]).abs()
+        dmax[d] = diff.permute(1, 0, 2, 3).reshape(npm, -1).amax(-1)
+        rmax[d] = Eref[:, d].abs().permute(1, 0, 2, 3).reshape(npm, -1).amax(-1)
+    row = rmax.amax(1, keepdim=True)
+    den = torch.where(rmax >= 1e-8 * row, rmax, row.expand_as(rmax))
+    err = dmax / den
+    return float(err.max()), float(err.mean())
+
+
+def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
+    """Reference loop nest (oracle/reference_port.py) timed on the host cores.
+
+    A full timeslice takes tens of minutes on a CPU, so the sample times a few
+    hop applications and a few (split, momentum) contractions of timeslice 0
+    and extrapolates with the exact operation counts of elemental.py:290-338:
+    per timeslice sum_d 2^{|d|} splits, each recomputing both displaced
+    vectors (|d| hops in total) and contracting once per momentum.
+    """
+    import numpy as np
+
+    from easydistillation_amd import synthetic as S
+    from oracle.elemental_oracle import derivative_list
+    from oracle.reference_port import ReferencePortGenerator
+
+    Lx, Ly, Lz, _ = latt
+    one = [Lx, Ly, Lz, 1]
+    rng = np.random.default_rng(0)
+    U = S.gauge_np(rng, one, "haar")
+    V = ((rng.standard_normal((1, Ne, Lz, Ly, Lx, 3)) + 1j * rng.standard_normal((1, Ne, Lz, Ly, Lx, 3))) / np.sqrt(6 * Lx * Ly * Lz)).astype(np.complex64)
+    gen = ReferencePortGenerator(one, U, V, num_nabla, moms)
+    Vt = V[0].astype(np.complex128)
+    dl = derivative_list(num_nabla)
+    n_splits = sum(2 ** len(d) for d in dl)
+    n_hops = sum(len(d) * 2 ** len(d) for d in dl)  # each split applies |d| hops in total
+    ph = gen._phase[tuple(moms[1 % len(moms)])]
+    # hops
+    t_hop, n_h, W = 0.0, 0, Vt
+    t0 = time.perf_counter()
+    while True:
+        W = gen._nD(Vt, U[:, 0], (n_h % 3,))
+        n_h += 1
+        t_hop = time.perf_counter() - t0
+        if n_h >= 2 and t_hop > seconds_target / 2:
+            break
+        if n_h >= 8:
+            break
+    # contractions
+    t_con, n_c = 0.0, 0
+    t0 = time.perf_counter()
+    while True:
+        np.einsum("zyx,ezyxc,fzyxc->ef", ph, W.conj(), Vt, optimize=True)
+        n_c += 1
+        t_con = time.perf_counter() - t0
+        if n_c >= 2 and t_con > seconds_target / 2:
+            break
+        if n_c >= 8:
+            break
+    per_t = n_hops * (t_hop / n_h) + n_splits * len(moms) * (t_con / n_c)
+    elementals_t = len(moms) * len(dl) * Ne * Ne
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": elementals_t / per_t,
+        "unit": "elementals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (
+            f"timeslice of {Lx}x{Ly}x{Lz}, Ne={Ne}: {n_h} hop applications ({t_hop / n_h:.2f} s each) and "
+            f"{n_c} split contractions ({t_con / n_c:.2f} s each) of the reference loop nest, extrapolated "
+            f"to its {n_hops} hops + {n_splits}x{len(moms)} contractions per timeslice ({per_t:.0f} s/t); "
+            f"numpy {np.__version__}, {threads} threads, {platform.processor() or platform.machine()}"
+        ),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS))
+    ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(
+            "bench.py measures one GPU only: elemental generation shards over timeslices with no "
+            "communication, and this build runs on single-GPU boxes; run with --gpus 1",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import torch
+
+    import easydistillation_amd as ed
+    from easydistillation_amd import _native
+    from easydistillation_amd import synthetic as S
+
+    latt, Ne, num_nabla, n2, kind = WORKLOADS[args.workload]
+    Lx, Ly, Lz, Lt = latt
+    moms = S.momenta_by_norm(n2)
+    D = ed.num_derivative(num_nabla)
+    P = len(moms)
+    dev = torch.device("cuda:0")
+    log(f"workload {args.workload}: latt={latt} Ne={Ne} D={D} P={P} links={kind}")
+
+    t0 = time.time()
+    U = S.gauge_torch(latt, kind, 0.3, seed=11, device=dev)
+    torch.cuda.synchronize()
+    log(f"links built in {time.time() - t0:.1f} s")
+    V = S.eigvecs_torch(latt, Ne, seed=12, device=dev, log=log)
+    torch.cuda.synchronize()
+    log(f"inputs built in {time.time() - t0:.1f} s")
+
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    outs = {}
+    for form in forms:
+        g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode=form)
+        g.load_arrays(U, V)
+        out = torch.empty((Lt, D, P, Ne, Ne), dtype=torch.complex128, device=dev)
+        for i in range(args.warmup):
+            t1 = time.perf_counter()
+            g.calc_range(0, Lt, out=out)
+            log(f"{form}: warmup {i} {time.perf_counter() - t1:.2f} s")
+        torch.cuda.synchronize()
+        g.context.set_option(_native.ED_OPT_PROFILE, 1)
+        g.context.profile_reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            g.calc_range(0, Lt, out=out)
+            log(f"{form}: step {i} done at {time.perf_counter() - t0:.2f} s")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        g.context.set_option(_native.ED_OPT_PROFILE, 0)
+        c_ms, c_n = g.context.profile_read(_native.ED_KERNEL_CONTRACT)
+        h_ms, h_n = g.context.profile_read(_native.ED_KERNEL_STENCIL)
+        s_ms, s_n = g.context.profile_read(_native.ED_KERNEL_STAGE)
+        ex_flops_t = g.context.query(_native.ED_Q_CONTRACT_FLOPS_PER_T)
+        algo_flops_t = g.context.query(_native.ED_Q_ALGO_FLOPS_PER_T)
+        hops_t = g.context.query(_native.ED_Q_HOPS_PER_T)
+        s = 16 if form == "fp64" else 8
+        hop_bytes_t = hops_t * Lx * Ly * Lz * (2 * Ne * 3 * s + 144)  # links are complex128 in both forms
+        results[form] = {
+            "ms_per_step": dt * 1e3,
+            "elementals_per_s": Lt * P * D * Ne * Ne / dt,
+            "contract_ms_per_launch": c_ms / max(c_n, 1),
+            "contract_launches_per_step": c_n // args.steps,
+            "contract_tflops_executed": ex_flops_t * Lt / (c_ms / args.steps * 1e-3) / 1e12,
+            "contract_tflops_algorithmic": algo_flops_t * Lt / (c_ms / args.steps * 1e-3) / 1e12,
+            "executed_flops_per_launch": ex_flops_t * Lt * args.steps / max(c_n, 1),
+            "stencil_gbs": hop_bytes_t * Lt / (h_ms / args.steps * 1e-3) / 1e9 if h_ms > 0 else None,
+            "stencil_ms_per_step": h_ms / args.steps,
+            "stage_ms_per_step": s_ms / args.steps,
+        }
+        log(f"{form}: {json.dumps(results[form])}")
+        outs[form] = out
+        del g
+
+    err = None
+    if "mixed" in outs and "fp64" in outs:
+        err = blockwise_error(outs["mixed"], outs["fp64"])
+        log(f"mixed vs fp64 blockwise error: max {err[0]:.3e} mean {err[1]:.3e}")
+    # V^dag V = 1 sanity check on the reference-numerics form
+    chk = outs.get("fp64", outs.get("mixed"))
+    eye_err = float((chk[:, 0, moms.index((0, 0, 0))] - torch.eye(Ne, dtype=chk.dtype, device=dev)).abs().max())
+    del outs, chk
+    torch.cuda.empty_cache()
+
+    head = "mixed" if "mixed" in results else forms[0]
+    r = results[head]
+    line = {
+        "metric": METRIC,
+        "value": r["elementals_per_s"],
+        "unit": "elementals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if head == "mixed" else "fp64",
+        "data": "synthetic (seeded Haar SU(3) links, orthonormal complex64 eigenvectors)",
+        "config": {
+            "workload": f"{Lx}^3x{Lt}" if Lx == Ly == Lz else f"{Lx}x{Ly}x{Lz}x{Lt}",
+            "Nev": Ne,
+            "momenta": P,
+            "derivatives": D,
+            "links": kind,
+            "form": head,
+            "elementals_per_step": Lt * P * D * Ne * Ne,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": r["contract_tflops_executed"],
+            "peak": PEAK[head],
+            "unit": "TFLOP/s",
+            "frac": r["contract_tflops_executed"] / PEAK[head],
+            "traffic": None,
+            "kernel": "contract_kernel<%s>" % ("float" if head == "mixed" else "double"),
+            "flops_basis": "executed MFMA flops of the factorized contraction per launch",
+            "executed_flops_per_launch": r["executed_flops_per_launch"],
+            "avg_launch_ms": r["contract_ms_per_launch"],
+            "algorithmic_equivalent_tflops": r["contract_tflops_algorithmic"],
+        },
+        "forms": results,
+        "mixed_vs_fp64_error": None if err is None else {"max": err[0], "mean": err[1], "metric": "per (d,p) max|dE|/max|E_fp64|"},
+        "identity_check_max_abs": eye_err,
+    }
+    if "fp64" in results and "mixed" in results:
+        line["speedup_mixed_vs_fp64"] = results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"]
+    if "fp64" in results and results["fp64"]["stencil_gbs"]:
+        line["stencil_roofline_fp64"] = {
+            "bound": "hbm",
+            "achieved": results["fp64"]["stencil_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": results["fp64"]["stencil_gbs"] / HBM_PEAK_GBS,
+        }
+    if not args.no_cpu_baseline:
+        log("cpu baseline sample ...")
+        line["cpu_baseline"] = cpu_baseline(latt, Ne, num_nabla, moms, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,42 @@
+"""MI355X-native distillation elemental generation.
+
+Drop-in for the ``ElementalGenerator`` path of EasyDistillation
+(lattice/generator/elemental.py) computing on gfx950 through the HIP library
+``lib/libelemental_hip.so`` (C ABI in ``include/elemental_hip.h``).
+"""
+
+from .backend import get_backend, set_backend
+from .constant import Nc, Nd, Ns
+from .elemental import ElementalGenerator, blending_weights
+from .insertion import MomentumPhase, derivative, derivative_index, num_derivative, output
+from .preset import (
+    ArrayFileData,
+    EigenvectorHostmem,
+    EigenvectorNpy,
+    ElementalNpy,
+    GaugeFieldHostmem,
+    GaugeFieldNpy,
+    save_elemental,
+)
+
+__all__ = [
+    "get_backend",
+    "set_backend",
+    "Nc",
+    "Nd",
+    "Ns",
+    "ElementalGenerator",
+    "blending_weights",
+    "MomentumPhase",
+    "derivative",
+    "derivative_index",
+    "num_derivative",
+    "output",
+    "ArrayFileData",
+    "EigenvectorHostmem",
+    "EigenvectorNpy",
+    "ElementalNpy",
+    "GaugeFieldHostmem",
+    "GaugeFieldNpy",
+    "save_elemental",
+]

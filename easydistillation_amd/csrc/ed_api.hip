@@ -1,0 +1,743 @@
+// ed_api.hip -- C ABI of the MI355X elemental-generation engine.
+// Declarations and conventions: include/elemental_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/elemental_hip.h"
+#include "ed_kernels.hpp"
+
+using ed::ContractArgs;
+using ed::DPlan;
+using ed::Geom;
+using ed::SplitTerm;
+
+namespace {
+
+struct EdError {
+  int code;
+  std::string msg;
+};
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      throw EdError{ED_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)};         \
+  } while (0)
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw EdError{code, buf};
+}
+
+// derivative(n) of lattice/insertion/derivative.py:23-33: length-k tuples
+// occupy [(3^k-1)/2, (3^(k+1)-1)/2); digits most significant first.
+std::vector<int> derivative_tuple(int n) {
+  int k = 0, p = 1;
+  while (n >= p) {
+    n -= p;
+    ++k;
+    p *= 3;
+  }
+  std::vector<int> d(k);
+  for (int i = k - 1; i >= 0; --i) {
+    d[i] = n % 3;
+    n /= 3;
+  }
+  return d;
+}
+
+int derivative_index(const std::vector<int>& d) {
+  int k = (int)d.size(), base = 0, p = 1;
+  for (int i = 0; i < k; ++i) {
+    base += p;
+    p *= 3;
+  }
+  int v = 0;
+  for (int x : d) v = v * 3 + x;
+  return base + v;
+}
+
+int level_start(int k) {  // (3^k - 1) / 2
+  int s = 0, p = 1;
+  for (int i = 0; i < k; ++i) {
+    s += p;
+    p *= 3;
+  }
+  return s;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    reset();
+    if (count == 0) return;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      fail(ED_ERR_ALLOC, "hipMalloc of %zu bytes failed", count * sizeof(T));
+    }
+    n = count;
+  }
+  ~DevBuf() { reset(); }
+};
+
+}  // namespace
+
+struct ed_context {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  int Lx = 0, Ly = 0, Lz = 0, Lt = 0, Ne = 0, V3 = 0, V3P = 0, NeP = 0;
+  bool lattice_set = false, gauge_set = false, eig_set = false, mom_set = false;
+  int num_nabla = 0, D = 1;
+  std::vector<int32_t> moms;  // [P][3]
+
+  DevBuf<double2> U64;
+  DevBuf<float2> V;
+  DevBuf<double> wgt;
+  bool use_wgt = false;
+
+  // plan
+  bool plan_dirty = true;
+  int hermitian = 1;
+  int nd = 0, npb = 1, nbase = 0, zero_idx = -1;
+  int64_t mfma_per_chunk_stage1 = 0;  // sum over primary d of 16 * 4 * nks(d)
+  int64_t total_splits = 0;           // split terms of elemental.py:309-321 over all d
+  DevBuf<DPlan> dplan;
+  DevBuf<SplitTerm> splits;
+  DevBuf<int> pidx;  // plus [npb*16] then minus [npb*16]
+  DevBuf<double> psum;  // [npb*16] sum over sites of Phi_b
+  DevBuf<double> ph64;
+  DevBuf<float> ph32;
+
+  // workspace
+  int tbatch_opt = 0;
+  int64_t ws_mb = 16384;
+  DevBuf<char> W;
+  DevBuf<double2> stage;
+
+  // profiling
+  int profile = 0;
+  double prof_ms[3] = {0, 0, 0};
+  int64_t prof_n[3] = {0, 0, 0};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  ~ed_context() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.second.first);
+      (void)hipEventDestroy(p.second.second);
+    }
+    for (auto ev : event_pool) (void)hipEventDestroy(ev);
+  }
+
+  Geom geom(int nW) const {
+    Geom g;
+    g.Lx = Lx;
+    g.Ly = Ly;
+    g.Lz = Lz;
+    g.Lt = Lt;
+    g.V3 = V3;
+    g.V3P = V3P;
+    g.Ne = Ne;
+    g.NeP = NeP;
+    g.nW = nW;
+    return g;
+  }
+
+  hipEvent_t get_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  }
+  void drain_events() {
+    for (auto& p : pending) {
+      float ms = 0;
+      HIPCHK(hipEventSynchronize(p.second.second));
+      HIPCHK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+      prof_ms[p.first] += ms;
+      prof_n[p.first] += 1;
+      event_pool.push_back(p.second.first);
+      event_pool.push_back(p.second.second);
+    }
+    pending.clear();
+  }
+  template <class F>
+  void launch(int kind, F&& f) {
+    if (!profile) {
+      f();
+      HIPCHK(hipGetLastError());
+      return;
+    }
+    hipEvent_t a = get_event(), b = get_event();
+    HIPCHK(hipEventRecord(a, stream));
+    f();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(b, stream));
+    pending.push_back({kind, {a, b}});
+  }
+};
+
+namespace {
+
+int guard(ed_context* ctx, const std::function<void()>& body);
+
+void require(bool c, int code, const char* msg) {
+  if (!c) fail(code, "%s", msg);
+}
+
+// ---------------------------------------------------------------------------
+// planning
+// ---------------------------------------------------------------------------
+void build_plan(ed_context* c) {
+  if (!c->plan_dirty) return;
+  require(c->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
+  require(c->mom_set, ED_ERR_STATE, "ed_set_momenta must be called first");
+  HIPCHK(hipSetDevice(c->device));
+  const int D = c->D;
+
+  // derivative plan: primary d and the partner rev(d) derived by Hermiticity
+  std::vector<DPlan> dpl;
+  std::vector<SplitTerm> sp;
+  c->mfma_per_chunk_stage1 = 0;
+  c->total_splits = 0;
+  for (int d = 0; d < D; ++d) {
+    std::vector<int> t = derivative_tuple(d);
+    c->total_splits += (int64_t)1 << t.size();
+    std::vector<int> r(t.rbegin(), t.rend());
+    const int rd = derivative_index(r);
+    int partner = -1;
+    if (c->hermitian && rd != d) {
+      if (rd < d) continue;  // written by its partner
+      partner = rd;
+    }
+    DPlan p{};
+    p.d = d;
+    p.partner = partner;
+    p.sign = (t.size() % 2) ? -1 : 1;
+    p.first = (int)sp.size();
+    // splits of elemental.py:309-321, identical (left, right) pairs merged
+    std::map<std::pair<int, int>, float> acc;
+    std::vector<std::pair<int, int>> order;
+    const int k = (int)t.size();
+    for (int pick = 0; pick < (1 << k); ++pick) {
+      std::vector<int> right, left;
+      for (int j = 0; j < k; ++j) ((pick >> j) & 1 ? right : left).push_back(t[j]);
+      std::reverse(left.begin(), left.end());
+      const std::pair<int, int> key{derivative_index(left), derivative_index(right)};
+      const float cf = (right.size() % 2) ? -1.f : 1.f;
+      if (!acc.count(key)) order.push_back(key);
+      acc[key] += cf;
+    }
+    for (auto& key : order) {
+      if (acc[key] == 0.f) continue;
+      SplitTerm s{};
+      s.nL = key.first;
+      s.nR = key.second;
+      s.coeff = acc[key];
+      sp.push_back(s);
+    }
+    p.nsplit = (int)sp.size() - p.first;
+    c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4 * ((3 * p.nsplit + 3) / 4);
+    dpl.push_back(p);
+  }
+  c->nd = (int)dpl.size();
+
+  // momentum plan: base momenta b (canonical sign: first non-zero component
+  // positive); E[+b] and E[-b] come from one contraction.  The first
+  // (0,0,0) goes to the FP64 VALU zero-momentum sum; repeated momenta get
+  // their own base entry.
+  const int P = (int)(c->moms.size() / 3);
+  struct Base {
+    int n[3];
+    int plus = -1, minus = -1;
+  };
+  std::vector<Base> base;
+  c->zero_idx = -1;
+  for (int p = 0; p < P; ++p) {
+    int n[3] = {c->moms[3 * p], c->moms[3 * p + 1], c->moms[3 * p + 2]};
+    int sgn = 0;
+    for (int i = 0; i < 3 && !sgn; ++i) sgn = n[i] > 0 ? 1 : (n[i] < 0 ? -1 : 0);
+    if (sgn == 0 && c->zero_idx < 0) {
+      c->zero_idx = p;
+      continue;
+    }
+    int cn[3] = {sgn < 0 ? -n[0] : n[0], sgn < 0 ? -n[1] : n[1], sgn < 0 ? -n[2] : n[2]};
+    bool placed = false;
+    for (auto& b : base) {
+      if (b.n[0] == cn[0] && b.n[1] == cn[1] && b.n[2] == cn[2]) {
+        int& slot = (sgn < 0) ? b.minus : b.plus;
+        if (slot < 0) {
+          slot = p;
+          placed = true;
+          break;
+        }
+      }
+    }
+    if (!placed) {
+      Base b;
+      b.n[0] = cn[0];
+      b.n[1] = cn[1];
+      b.n[2] = cn[2];
+      ((sgn < 0) ? b.minus : b.plus) = p;
+      base.push_back(b);
+    }
+  }
+  c->nbase = (int)base.size();
+  c->npb = std::max(1, (c->nbase + 15) / 16);
+  const int NB = c->npb * 16;
+  std::vector<int> idx(2 * NB, -1);
+  for (int i = 0; i < c->nbase; ++i) {
+    idx[i] = base[i].plus;
+    idx[NB + i] = base[i].minus;
+  }
+  // FP64 phase table [npb][V3P][16]; exp(2 pi i frac) with the integer
+  // products reduced modulo L first (phase.py:11-13,41-46 up to rounding)
+  std::vector<double> pr((size_t)NB * c->V3P, 0.0), pi((size_t)NB * c->V3P, 0.0);
+  for (int i = 0; i < c->nbase; ++i) {
+    const int blk = i / 16, col = i % 16;
+    for (int z = 0; z < c->Lz; ++z)
+      for (int y = 0; y < c->Ly; ++y)
+        for (int x = 0; x < c->Lx; ++x) {
+          auto md = [](long a, long m) { long r = a % m; return r < 0 ? r + m : r; };
+          const double fr = (double)md((long)base[i].n[0] * x, c->Lx) / c->Lx +
+                            (double)md((long)base[i].n[1] * y, c->Ly) / c->Ly +
+                            (double)md((long)base[i].n[2] * z, c->Lz) / c->Lz;
+          const double ang = 2.0 * M_PI * fr;
+          const size_t s = ((size_t)z * c->Ly + y) * c->Lx + x;
+          pr[((size_t)blk * c->V3P + s) * 16 + col] = std::cos(ang);
+          pi[((size_t)blk * c->V3P + s) * 16 + col] = std::sin(ang);
+        }
+  }
+  std::vector<float> pr32(pr.begin(), pr.end()), pi32(pi.begin(), pi.end());
+  // sum_x Phi_b(x) is V3 when every component of b is a multiple of L, else 0
+  std::vector<double> ps(NB, 0.0);
+  for (int i = 0; i < c->nbase; ++i)
+    if (base[i].n[0] % c->Lx == 0 && base[i].n[1] % c->Ly == 0 && base[i].n[2] % c->Lz == 0) ps[i] = (double)c->V3;
+
+  c->dplan.ensure(dpl.size());
+  c->splits.ensure(std::max<size_t>(1, sp.size()));
+  c->pidx.ensure(idx.size());
+  c->psum.ensure(ps.size());
+  c->ph64.ensure(2 * pr.size());
+  c->ph32.ensure(2 * pr.size());
+  HIPCHK(hipMemcpy(c->dplan.p, dpl.data(), dpl.size() * sizeof(DPlan), hipMemcpyHostToDevice));
+  if (!sp.empty()) HIPCHK(hipMemcpy(c->splits.p, sp.data(), sp.size() * sizeof(SplitTerm), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->pidx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->psum.p, ps.data(), ps.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ph64.p, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ph64.p + pr.size(), pi.data(), pi.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ph32.p, pr32.data(), pr32.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ph32.p + pr32.size(), pi32.data(), pi32.size() * sizeof(float), hipMemcpyHostToDevice));
+  c->plan_dirty = false;
+}
+
+__global__ void c128_to_c64_kernel(const double2* __restrict__ in, float2* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+__global__ void c64_to_c128_kernel(const float2* __restrict__ in, double2* __restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = make_double2((double)in[i].x, (double)in[i].y);
+}
+
+template <typename T>
+size_t per_t_ws_bytes(const ed_context* c) {
+  return (size_t)c->D * c->NeP * 3 * c->V3P * sizeof(ed::c_t<T>);
+}
+
+template <typename T>
+void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host) {
+  const T* ph = nullptr;
+  if (sizeof(T) == 8) ph = (const T*)(void*)c->ph64.p;
+  else ph = (const T*)(void*)c->ph32.p;
+  const size_t phn = (size_t)c->npb * 16 * c->V3P;
+  const double2* U = c->U64.p;
+
+  const size_t per_t = per_t_ws_bytes<T>(c);
+  int TB = c->tbatch_opt;
+  if (TB <= 0) {
+    const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    TB = (int)std::max<size_t>(1, budget / per_t);
+  }
+  TB = std::min(TB, nt);
+  const size_t ws = per_t * TB;
+  if (c->W.n < ws) {
+    c->W.ensure(ws);
+    // padded rows and sites must hold zeros; they are never written later
+    HIPCHK(hipMemsetAsync(c->W.p, 0, ws, c->stream));
+  }
+  ed::c_t<T>* W = (ed::c_t<T>*)(void*)c->W.p;
+  const Geom g = c->geom(c->D);
+  const size_t out_t = (size_t)c->D * (c->moms.size() / 3) * c->Ne * c->Ne;
+
+  double2* out = out_dev;
+  if (!out) {
+    c->stage.ensure(out_t * TB);
+    out = c->stage.p;
+  }
+
+  for (int tb0 = t0; tb0 < t0 + nt; tb0 += TB) {
+    const int nb = std::min(TB, t0 + nt - tb0);
+    // V(t) -> W slot 0
+    c->launch(ED_KERNEL_STAGE, [&] {
+      dim3 grid((3 * c->V3 + 255) / 256, c->Ne, nb);
+      ed::stage_v_kernel<T><<<grid, 256, 0, c->stream>>>(c->V.p, W, g, tb0);
+    });
+    // covariant hops level by level
+    const int echunk = 8, nech = (c->Ne + echunk - 1) / echunk;
+    for (int k = 1; k <= c->num_nabla; ++k) {
+      const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
+      const int child0 = level_start(k);
+      c->launch(ED_KERNEL_STENCIL, [&] {
+        dim3 grid((c->V3 + 255) / 256, 3 * nparent, nb * nech);
+        ed::hop_kernel<T><<<grid, 256, 0, c->stream>>>(W, U, g, tb0, parent0, child0, echunk, nech);
+      });
+    }
+    // contraction
+    ContractArgs a{};
+    a.g = g;
+    a.D = c->D;
+    a.P = (int)(c->moms.size() / 3);
+    a.nd = c->nd;
+    a.npb = c->npb;
+    a.nbase = c->nbase;
+    a.nte = c->NeP / 16;
+    a.ntf = c->NeP / 16;
+    a.nwork = nb * a.nd * a.npb * a.nte * a.ntf;
+    a.zero_idx = c->zero_idx;
+    a.t_out0 = out_dev ? (tb0 - t0) : 0;
+    a.dplan = c->dplan.p;
+    a.splits = c->splits.p;
+    a.plus_idx = c->pidx.p;
+    a.minus_idx = c->pidx.p + c->npb * 16;
+    a.wgt = c->use_wgt ? c->wgt.p : nullptr;
+    a.psum = c->psum.p;
+    const int nblk = (a.nwork + 7) / 8 * 8;
+    c->launch(ED_KERNEL_CONTRACT, [&] {
+      ed::contract_kernel<T><<<nblk, 256, 0, c->stream>>>(W, ph, ph + phn, out, a);
+    });
+    if (!out_dev) {
+      HIPCHK(hipMemcpyAsync((double2*)out_host + (size_t)(tb0 - t0) * out_t, out, (size_t)nb * out_t * sizeof(double2),
+                            hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  }
+  if (c->profile) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->drain_events();
+  }
+}
+
+int guard(ed_context* ctx, const std::function<void()>& body) {
+  try {
+    body();
+    if (ctx) ctx->err.clear();
+    return ED_OK;
+  } catch (const EdError& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return ED_ERR_ALLOC;
+  } catch (...) {
+    if (ctx) ctx->err = "unknown error";
+    return ED_ERR_ARG;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ed_version(void) { return "elemental_hip 0.1 (gfx950)"; }
+
+int ed_create(int device, ed_context** ctx) {
+  if (!ctx) return ED_ERR_ARG;
+  *ctx = nullptr;
+  ed_context* c = new (std::nothrow) ed_context();
+  if (!c) return ED_ERR_ALLOC;
+  int rc = guard(c, [&] {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(ED_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    c->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+    c->stream = c->own;
+  });
+  if (rc != ED_OK) {
+    delete c;
+    return rc;
+  }
+  *ctx = c;
+  return ED_OK;
+}
+
+int ed_destroy(ed_context* ctx) {
+  if (!ctx) return ED_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  delete ctx;
+  return ED_OK;
+}
+
+const char* ed_last_error(const ed_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ed_set_stream(ed_context* ctx, void* s) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] { ctx->stream = s ? (hipStream_t)s : ctx->own; });
+}
+
+int ed_set_lattice(ed_context* ctx, int Lx, int Ly, int Lz, int Lt, int Ne) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (Lx < 1 || Ly < 1 || Lz < 1 || Lt < 1 || Ne < 1) fail(ED_ERR_ARG, "lattice sizes and Ne must be >= 1");
+    const long v3 = (long)Lx * Ly * Lz;
+    if (v3 > (1L << 26)) fail(ED_ERR_ARG, "spatial volume %ld too large", v3);
+    if (Ne > 3 * v3) fail(ED_ERR_ARG, "Ne = %d exceeds 3*V3 = %ld", Ne, 3 * v3);
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->Lx = Lx;
+    ctx->Ly = Ly;
+    ctx->Lz = Lz;
+    ctx->Lt = Lt;
+    ctx->Ne = Ne;
+    ctx->V3 = (int)v3;
+    ctx->V3P = (int)((v3 + ed::SC - 1) / ed::SC * ed::SC);
+    ctx->NeP = (Ne + 15) / 16 * 16;
+    ctx->lattice_set = true;
+    ctx->gauge_set = ctx->eig_set = false;
+    ctx->use_wgt = false;
+    ctx->plan_dirty = true;
+    ctx->W.reset();
+    ctx->stage.reset();
+  });
+}
+
+int ed_set_gauge(ed_context* ctx, const void* U, int dtype, int on_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
+    if (!U) fail(ED_ERR_ARG, "U is null");
+    if (dtype != ED_C128 && dtype != ED_C64) fail(ED_ERR_ARG, "dtype must be ED_C128 or ED_C64");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)3 * ctx->Lt * ctx->V3 * 9;
+    ctx->U64.ensure(n);
+    if (dtype == ED_C128) {
+      HIPCHK(hipMemcpyAsync(ctx->U64.p, U, n * sizeof(double2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            ctx->stream));
+    } else {
+      DevBuf<float2> tmp;
+      tmp.ensure(n);
+      HIPCHK(hipMemcpyAsync(tmp.p, U, n * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            ctx->stream));
+      c64_to_c128_kernel<<<1024, 256, 0, ctx->stream>>>(tmp.p, ctx->U64.p, n);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->gauge_set = true;
+  });
+}
+
+int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
+    if (!V) fail(ED_ERR_ARG, "V is null");
+    if (dtype != ED_C128 && dtype != ED_C64) fail(ED_ERR_ARG, "dtype must be ED_C128 or ED_C64");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->Lt * ctx->Ne * ctx->V3 * 3;
+    ctx->V.ensure(n);
+    if (dtype == ED_C64) {
+      HIPCHK(hipMemcpyAsync(ctx->V.p, V, n * sizeof(float2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            ctx->stream));
+    } else {
+      // round complex128 to complex64 (the reference's c8 staging buffer)
+      const size_t chunk = (size_t)1 << 24;
+      DevBuf<double2> tmp;
+      tmp.ensure(std::min(n, chunk));
+      for (size_t s = 0; s < n; s += chunk) {
+        const size_t m = std::min(chunk, n - s);
+        HIPCHK(hipMemcpyAsync(tmp.p, (const double2*)V + s, m * sizeof(double2),
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+        c128_to_c64_kernel<<<1024, 256, 0, ctx->stream>>>(tmp.p, ctx->V.p + s, m);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->eig_set = true;
+  });
+}
+
+int ed_set_momenta(ed_context* ctx, const int32_t* n, int P) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (P < 1 || !n) fail(ED_ERR_ARG, "momentum list must be non-empty");
+    if (P > 4096) fail(ED_ERR_ARG, "at most 4096 momenta");
+    ctx->moms.assign(n, n + 3 * P);
+    ctx->mom_set = true;
+    ctx->plan_dirty = true;
+  });
+}
+
+int ed_set_derivatives(ed_context* ctx, int num_nabla) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (num_nabla < 0 || num_nabla > 3) fail(ED_ERR_ARG, "num_nabla must be in [0, 3]");
+    ctx->num_nabla = num_nabla;
+    ctx->D = level_start(num_nabla + 1);
+    ctx->plan_dirty = true;
+    ctx->W.reset();
+    ctx->stage.reset();
+  });
+}
+
+int ed_set_weights(ed_context* ctx, const double* coeff) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!coeff) {
+      ctx->use_wgt = false;
+      return;
+    }
+    const size_t n = (size_t)ctx->Ne * ctx->Ne;
+    ctx->wgt.ensure(n);
+    HIPCHK(hipMemcpy(ctx->wgt.p, coeff, n * sizeof(double), hipMemcpyHostToDevice));
+    ctx->use_wgt = true;
+  });
+}
+
+int ed_set_option(ed_context* ctx, int option, int64_t value) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    switch (option) {
+      case ED_OPT_HERMITIAN:
+        ctx->hermitian = value ? 1 : 0;
+        ctx->plan_dirty = true;
+        break;
+      case ED_OPT_TBATCH:
+        if (value < 0) fail(ED_ERR_ARG, "batch must be >= 0");
+        ctx->tbatch_opt = (int)value;
+        break;
+      case ED_OPT_WS_MB:
+        if (value < 1) fail(ED_ERR_ARG, "workspace budget must be >= 1 MiB");
+        ctx->ws_mb = value;
+        break;
+      case ED_OPT_PROFILE:
+        ctx->profile = value ? 1 : 0;
+        break;
+      default:
+        fail(ED_ERR_ARG, "unknown option %d", option);
+    }
+  });
+}
+
+int ed_compute(ed_context* ctx, int t0, int nt, int mode, void* out, int out_on_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set && ctx->gauge_set && ctx->eig_set && ctx->mom_set, ED_ERR_STATE,
+            "lattice, gauge, eigenvectors and momenta must be set before ed_compute");
+    if (!out) fail(ED_ERR_ARG, "out is null");
+    if (t0 < 0 || nt < 1 || t0 + nt > ctx->Lt) fail(ED_ERR_ARG, "timeslice range [%d, %d) outside [0, %d)", t0, t0 + nt, ctx->Lt);
+    if (mode != ED_MODE_FP64 && mode != ED_MODE_MIXED) fail(ED_ERR_ARG, "unknown mode %d", mode);
+    HIPCHK(hipSetDevice(ctx->device));
+    build_plan(ctx);
+    if (mode == ED_MODE_FP64)
+      run_batches<double>(ctx, t0, nt, out_on_device ? (double2*)out : nullptr, out_on_device ? nullptr : out);
+    else
+      run_batches<float>(ctx, t0, nt, out_on_device ? (double2*)out : nullptr, out_on_device ? nullptr : out);
+  });
+}
+
+int ed_query(ed_context* ctx, int what, int64_t* value) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (!value) fail(ED_ERR_ARG, "value is null");
+    build_plan(ctx);
+    const int64_t tiles = (int64_t)(ctx->NeP / 16) * (ctx->NeP / 16);
+    const int64_t chunks = ctx->V3P / ed::SC;
+    const int64_t P = (int64_t)(ctx->moms.size() / 3);
+    switch (what) {
+      case ED_Q_CONTRACT_FLOPS_PER_T: {
+        // per work item and chunk: stage 1 (16 sites x 4 MFMAs per k-step) and
+        // stage 2 (4 waves x 4 k-steps x 4 N-blocks x 4 real products)
+        const int64_t mf = ctx->npb * tiles * chunks * (ctx->mfma_per_chunk_stage1 + (int64_t)ctx->nd * 256);
+        *value = mf * 2 * 16 * 16 * 4;
+        break;
+      }
+      case ED_Q_ALGO_FLOPS_PER_T:
+        *value = 8 * (int64_t)ctx->Ne * ctx->Ne * 3 * ctx->V3 * ctx->total_splits * P;
+        break;
+      case ED_Q_PRIMARY_DERIVATIVES: *value = ctx->nd; break;
+      case ED_Q_MOMENTUM_BLOCKS: *value = ctx->npb; break;
+      case ED_Q_HOPS_PER_T: *value = ctx->D - 1; break;
+      case ED_Q_SPLITS: *value = ctx->total_splits; break;
+      default: fail(ED_ERR_ARG, "unknown query %d", what);
+    }
+  });
+}
+
+int ed_synchronize(ed_context* ctx) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->drain_events();
+  });
+}
+
+int ed_profile_read(ed_context* ctx, int kernel, double* total_ms, int64_t* launches) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (kernel < 0 || kernel > 2) fail(ED_ERR_ARG, "unknown kernel family %d", kernel);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->drain_events();
+    if (total_ms) *total_ms = ctx->prof_ms[kernel];
+    if (launches) *launches = ctx->prof_n[kernel];
+  });
+}
+
+int ed_profile_reset(ed_context* ctx) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->drain_events();
+    for (int k = 0; k < 3; ++k) {
+      ctx->prof_ms[k] = 0;
+      ctx->prof_n[k] = 0;
+    }
+  });
+}
+
+}  // extern "C"

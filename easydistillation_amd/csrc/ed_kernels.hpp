@@ -1,0 +1,411 @@
+// ed_kernels.hpp -- gfx950 device code for elemental generation.
+//
+// Data layout in HBM (one timeslice batch of TB slices):
+//   W  [TB][nW][NeP][V3P][3]  complex<T>, nW = D displaced vectors
+//      W_n = nabla_{d_k} ... nabla_{d_1} V for derivative(n) = (d_1..d_k)
+//      (lattice/generator/elemental.py:279-288 applies tuple elements in
+//      order); NeP = Ne rounded up to 16, V3P = V3 rounded up to 16; padded
+//      rows and sites hold zeros so the contraction needs no tail masks.
+//   U  [3][Lt][V3][3][3]      complex<double> spatial links (elemental.py:103)
+//   Vd [Lt][Ne][V3][3]        complex<float>: the reference stages V in a
+//                             complex64 buffer (elemental.py:55)
+//   PH [NPB][V3P][16] x2      real and imaginary phase tables of the base
+//                             momenta (one block of 16 per NPB)
+//   out[nt][D][P][Ne][Ne]     complex<double>
+//
+// The contraction evaluates, for every derivative d and base momentum b,
+//   Q_d[e,f,x]  = sum_{splits s} c_s sum_c conj(W_{L_s}[e,x,c]) W_{R_s}[f,x,c]   (stage 1)
+//   E_d[+b]     = sum_x Phi_b(x) Q_d[e,f,x],  E_d[-b] = sum_x conj(Phi_b(x)) Q_d[e,f,x] (stage 2)
+// which equals the reference's sum over splits and momenta of
+// contract("zyx,ezyxc,fzyxc->ef", c_s Phi_p, left.conj(), right)
+// (elemental.py:309-329): stage 2 is linear in Q, so the split sum is taken
+// per site before the phase contraction, and the four real products
+// Re/Im(Phi) x Re/Im(Q) serve +b and -b at once.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ed {
+
+template <typename T> struct cplx;
+template <> struct cplx<double> { using type = double2; };
+template <> struct cplx<float> { using type = float2; };
+template <typename T> using c_t = typename cplx<T>::type;
+
+template <typename T> __device__ __forceinline__ c_t<T> mk(T r, T i) { c_t<T> v; v.x = r; v.y = i; return v; }
+
+// --------------------------------------------------------------------------
+// MFMA wrappers: 16x16x4, one operand value per lane.
+//   A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k (both types).
+//   D[i][j] sits in lane j + 16 g, element r, with
+//     f32: i = 4 g + r      f64: i = 4 r + g
+//   (measured on gfx950 with one-hot operands); row(g, r) returns i.
+// --------------------------------------------------------------------------
+template <typename T> struct mfma;
+template <> struct mfma<double> {
+  typedef double acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mac(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
+};
+template <> struct mfma<float> {
+  typedef float acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mac(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+
+struct Geom {
+  int Lx, Ly, Lz, Lt;
+  int V3, V3P, Ne, NeP;
+  int nW;
+};
+
+// --------------------------------------------------------------------------
+// Stage V(t) (complex64) into slot 0 of W in the compute precision.
+// grid (ceil(3 V3 / 256), Ne, TB)
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void stage_v_kernel(const float2* __restrict__ V, c_t<T>* __restrict__ W,
+                                                       Geom g, int tb0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y, tb = blockIdx.z;
+  if (i >= 3 * g.V3) return;
+  const float2 v = V[((size_t)(tb0 + tb) * g.Ne + e) * (size_t)(3 * g.V3) + i];
+  W[(((size_t)tb * g.nW) * g.NeP + e) * (size_t)(3 * g.V3P) + i] = mk<T>((T)v.x, (T)v.y);
+}
+
+// --------------------------------------------------------------------------
+// Covariant hop (elemental.py:282-287), periodic:
+//   out(x) = U_mu(x) in(x + mu) - U_mu(x - mu)^dag in(x - mu)
+// One launch computes every child of one derivative level: blockIdx.y
+// enumerates (parent, mu); parent = parent0 + y / 3, child = child0 + y.
+// A thread owns one site, keeps both links in registers and loops over a
+// chunk of eigenvectors, so U is read once per chunk and V/W reads along x
+// are contiguous across the wave.  Arithmetic is FP64 for both forms; the
+// complex64 form rounds once, when the result is stored.
+// grid (ceil(V3 / 256), 3 * nparent, TB * nechunk)
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void hop_kernel(c_t<T>* __restrict__ W, const double2* __restrict__ U, Geom g,
+                                                   int tb0, int parent0, int child0, int echunk, int nechunk) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.V3) return;
+  const int y = blockIdx.y;
+  const int parent = parent0 + y / 3, mu = y % 3, child = child0 + y;
+  const int tb = blockIdx.z / nechunk, ec = blockIdx.z % nechunk;
+  const int x = s % g.Lx, yy = (s / g.Lx) % g.Ly, z = s / (g.Lx * g.Ly);
+  int sf, sb;
+  if (mu == 0) {
+    sf = s - x + (x + 1 == g.Lx ? 0 : x + 1);
+    sb = s - x + (x == 0 ? g.Lx - 1 : x - 1);
+  } else if (mu == 1) {
+    const int r = s - yy * g.Lx;
+    sf = r + (yy + 1 == g.Ly ? 0 : yy + 1) * g.Lx;
+    sb = r + (yy == 0 ? g.Ly - 1 : yy - 1) * g.Lx;
+  } else {
+    const int r = s - z * g.Lx * g.Ly;
+    sf = r + (z + 1 == g.Lz ? 0 : z + 1) * g.Lx * g.Ly;
+    sb = r + (z == 0 ? g.Lz - 1 : z - 1) * g.Lx * g.Ly;
+  }
+  const double2* Um = U + ((size_t)mu * g.Lt + (tb0 + tb)) * (size_t)g.V3 * 9;
+  double2 uf[9], ub[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    uf[k] = Um[(size_t)s * 9 + k];
+    ub[k] = Um[(size_t)sb * 9 + k];
+  }
+  const size_t row = (size_t)3 * g.V3P;
+  const c_t<T>* in = W + ((size_t)tb * g.nW + parent) * g.NeP * row;
+  c_t<T>* out = W + ((size_t)tb * g.nW + child) * g.NeP * row;
+  const int e_end = min(g.Ne, (ec + 1) * echunk);
+  for (int e = ec * echunk; e < e_end; ++e) {
+    const c_t<T>* ie = in + (size_t)e * row;
+    double2 af[3], ab[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const c_t<T> vf = ie[(size_t)sf * 3 + b], vb = ie[(size_t)sb * 3 + b];
+      af[b] = make_double2(vf.x, vf.y);
+      ab[b] = make_double2(vb.x, vb.y);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double re = 0, im = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double2 u = uf[a * 3 + b];
+        re += u.x * af[b].x - u.y * af[b].y;
+        im += u.x * af[b].y + u.y * af[b].x;
+        const double2 v = ub[b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
+        re -= v.x * ab[b].x + v.y * ab[b].y;
+        im -= v.x * ab[b].y - v.y * ab[b].x;
+      }
+      out[(size_t)e * row + (size_t)s * 3 + a] = mk<T>((T)re, (T)im);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Contraction.
+// --------------------------------------------------------------------------
+struct SplitTerm {
+  int nL, nR;
+  float coeff;  // (-1)^{#right}, merged duplicates add up
+  int pad;
+};
+
+struct DPlan {
+  int d;            // output derivative index
+  int partner;      // rev(d) written from this d's result, or -1
+  int sign;         // (-1)^{|d|} for the partner
+  int first, nsplit;
+  int pad[3];
+};
+
+struct ContractArgs {
+  Geom g;
+  int D, P;
+  int nd;         // primary derivative count
+  int npb;        // momentum blocks of 16 base momenta
+  int nbase;      // base momenta used (<= 16 npb)
+  int nte, ntf;   // e / f tiles of 16
+  int nwork;      // TB * nd * npb * nte * ntf
+  int zero_idx;   // output index of p = (0,0,0) handled on VALU, or -1
+  int t_out0;     // batch offset inside out (in timeslices)
+  const DPlan* dplan;
+  const SplitTerm* splits;
+  const int* plus_idx;   // [npb*16] output index of +b or -1
+  const int* minus_idx;  // [npb*16] output index of -b or -1
+  const double* wgt;     // [Ne][Ne] blending weights or nullptr
+  const double* psum;    // [npb*16] sum_x Phi_b(x): V3 when b = 0 mod L, else 0
+};
+
+constexpr int SC = 16;  // sites per chunk (4 per wave in stage 1, 4 k-steps in stage 2)
+
+// Workgroup id -> work item such that each XCD (workgroups are dispatched
+// round-robin over 8 XCDs) receives a contiguous range of work items, so the
+// tiles of one (t, d) share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int per = nblocks >> 3;  // nblocks is a multiple of 8
+  return (bid & 7) * per + (bid >> 3);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, 1) void contract_kernel(const c_t<T>* __restrict__ W, const T* __restrict__ phr,
+                                                           const T* __restrict__ phi, double2* __restrict__ out,
+                                                           ContractArgs a) {
+  typedef typename mfma<T>::acc acc_t;
+  __shared__ c_t<T> Qs[SC][16][16];
+
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= a.nwork) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g4 = lane >> 4;
+
+  int rest = w;
+  const int tf = rest % a.ntf; rest /= a.ntf;
+  const int te = rest % a.nte; rest /= a.nte;
+  const int pb = rest % a.npb; rest /= a.npb;
+  const int di = rest % a.nd;
+  const int tb = rest / a.nd;
+  const DPlan dp = a.dplan[di];
+  const int E0 = te * 16, F0 = tf * 16;
+  const Geom g = a.g;
+  const size_t row = (size_t)3 * g.V3P;
+  const c_t<T>* Wt = W + (size_t)tb * g.nW * g.NeP * row;
+  const int J = 3 * dp.nsplit;
+  const int nks = (J + 3) >> 2;
+
+  acc_t A1[4], A2[4], A3[4], A4[4];
+  double D1[4][4], D2[4][4], D3[4][4], D4[4][4];  // FP64 cross-chunk sums (used when T = float)
+  T zr[4], zi[4];
+  double Zr[4], Zi[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
+    zr[b] = zi[b] = 0;
+    Zr[b] = Zi[b] = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
+  }
+
+  const T* PR = phr + (size_t)pb * g.V3P * 16;
+  const T* PI = phi + (size_t)pb * g.V3P * 16;
+
+  // FP32 form: Q_d[e,f,x] often carries a large site-independent part (for
+  // d = 0, |V_e(x)|^2 >= 0).  Stage 2 works on Q - c with c the mean over the
+  // first chunk, and adds c * sum_x Phi_b(x) back in FP64, so FP32 rounding
+  // scales with the fluctuation of Q rather than with its mean.
+  T cr[4] = {0, 0, 0, 0}, ci[4] = {0, 0, 0, 0};
+
+  for (int c0 = 0; c0 < g.V3P; c0 += SC) {
+    // ---- stage 1: Q_d[e, f, x] for the 4 sites of this wave ----
+    acc_t qr[4], qi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) qr[i] = qi[i] = acc_t{0, 0, 0, 0};
+    const int xb = c0 + 4 * wave;
+    for (int ks = 0; ks < nks; ++ks) {
+      const int j = 4 * ks + g4;
+      T lr[4], li[4], rr[4], ri[4];
+      if (j < J) {
+        const SplitTerm st = a.splits[dp.first + j / 3];
+        const int c = j % 3;
+        const c_t<T>* Lp = Wt + ((size_t)st.nL * g.NeP + E0 + l16) * row + c;
+        const c_t<T>* Rp = Wt + ((size_t)st.nR * g.NeP + F0 + l16) * row + c;
+        const T cf = (T)st.coeff;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const c_t<T> lv = Lp[(size_t)(xb + i) * 3];
+          const c_t<T> rv = Rp[(size_t)(xb + i) * 3];
+          lr[i] = cf * lv.x;
+          li[i] = cf * lv.y;
+          rr[i] = rv.x;
+          ri[i] = rv.y;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lr[i] = li[i] = rr[i] = ri[i] = 0;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // conj(L) R = (lr rr + li ri) + i (lr ri - li rr)
+        qr[i] = mfma<T>::mac(lr[i], rr[i], qr[i]);
+        qr[i] = mfma<T>::mac(li[i], ri[i], qr[i]);
+        qi[i] = mfma<T>::mac(lr[i], ri[i], qi[i]);
+        qi[i] = mfma<T>::mac(-li[i], rr[i], qi[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Qs[4 * wave + i][mfma<T>::row(g4, r)][l16] = mk<T>(qr[i][r], qi[i][r]);
+    __syncthreads();
+
+    if constexpr (sizeof(T) == 4) {
+      if (c0 == 0) {
+        const int n0 = min(SC, g.V3);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          T sr = 0, si = 0;
+          for (int xl = 0; xl < n0; ++xl) {
+            const c_t<T> q = Qs[xl][4 * wave + b][l16];
+            sr += q.x;
+            si += q.y;
+          }
+          cr[b] = sr / (T)n0;
+          ci[b] = si / (T)n0;
+        }
+      }
+    }
+
+    // ---- stage 2: phase contraction over the 16 sites of the chunk ----
+#pragma unroll
+    for (int k = 0; k < SC / 4; ++k) {
+      const int xl = 4 * k + g4;
+      const size_t xo = (size_t)(c0 + xl) * 16 + l16;
+      const T pr = PR[xo], pi = PI[xo];
+      const bool real_site = (c0 + xl) < g.V3;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        c_t<T> q = Qs[xl][4 * wave + b][l16];
+        if constexpr (sizeof(T) == 4) {
+          q.x = real_site ? q.x - cr[b] : (T)0;
+          q.y = real_site ? q.y - ci[b] : (T)0;
+        }
+        A1[b] = mfma<T>::mac(pr, q.x, A1[b]);
+        A2[b] = mfma<T>::mac(pi, q.y, A2[b]);
+        A3[b] = mfma<T>::mac(pi, q.x, A3[b]);
+        A4[b] = mfma<T>::mac(pr, q.y, A4[b]);
+        zr[b] += q.x;
+        zi[b] += q.y;
+      }
+    }
+    __syncthreads();
+
+    if (sizeof(T) == 4) {
+      // FP32 partial sums of one chunk are moved into FP64 accumulators.
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          D1[b][r] += (double)A1[b][r];
+          D2[b][r] += (double)A2[b][r];
+          D3[b][r] += (double)A3[b][r];
+          D4[b][r] += (double)A4[b][r];
+        }
+        A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
+        Zr[b] += (double)zr[b];
+        Zi[b] += (double)zi[b];
+        zr[b] = zi[b] = 0;
+      }
+    }
+  }
+  if (sizeof(T) == 8) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        D1[b][r] = (double)A1[b][r];
+        D2[b][r] = (double)A2[b][r];
+        D3[b][r] = (double)A3[b][r];
+        D4[b][r] = (double)A4[b][r];
+      }
+      Zr[b] = (double)zr[b];
+      Zi[b] = (double)zi[b];
+    }
+  }
+
+  // ---- epilogue ----
+  const int Ne = g.Ne, P = a.P;
+  const size_t plane = (size_t)Ne * Ne;
+  const size_t tbase = (size_t)(a.t_out0 + tb) * a.D * P;
+  const int f = F0 + l16;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int e = E0 + 4 * wave + b;
+    const bool ok = (e < Ne) && (f < Ne);
+    const double wef = (ok && a.wgt) ? a.wgt[(size_t)e * Ne + f] : 1.0;
+    const double wfe = (ok && a.wgt) ? a.wgt[(size_t)f * Ne + e] : 1.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int bi = pb * 16 + mfma<T>::row(g4, r);
+      if (!ok || bi >= a.nbase) continue;
+      const int ip = a.plus_idx[bi], im = a.minus_idx[bi];
+      const double S = a.psum[bi], mr = (double)cr[b] * S, mi = (double)ci[b] * S;
+      const double epr = D1[b][r] - D2[b][r] + mr, epi = D3[b][r] + D4[b][r] + mi;  // E_d[+b]
+      const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;  // E_d[-b]
+      if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
+      if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
+      if (dp.partner >= 0) {
+        // E_{rev d}[p'] = sign * E_d[-p']^dagger
+        const double s = (double)dp.sign;
+        if (im >= 0)
+          out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
+              make_double2(s * epr * wfe, -s * epi * wfe);
+        if (ip >= 0)
+          out[(tbase + (size_t)dp.partner * P + ip) * plane + (size_t)f * Ne + e] =
+              make_double2(s * emr * wfe, -s * emi * wfe);
+      }
+    }
+    // zero momentum: sum the four row groups (sites 4k + g4) of each lane column
+    double zre = Zr[b], zim = Zi[b];
+    zre += __shfl_xor(zre, 16);
+    zim += __shfl_xor(zim, 16);
+    zre += __shfl_xor(zre, 32);
+    zim += __shfl_xor(zim, 32);
+    zre += (double)cr[b] * g.V3;
+    zim += (double)ci[b] * g.V3;
+    if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
+      out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
+            make_double2(s * zre * wfe, -s * zim * wfe);
+      }
+    }
+  }
+}
+
+}  // namespace ed

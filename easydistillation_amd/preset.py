@@ -1,0 +1,129 @@
+"""Input/output containers with the reference's ``load(key)`` contract.
+
+``ElementalGenerator.load(key)`` needs (lattice/generator/elemental.py:102-105,
+lattice/filedata/abstract.py:12-20):
+
+* ``gauge_field.load(key)[:]`` -> array [Lt, Lz, Ly, Lx, Nd, Nc, Nc];
+* ``eigenvector.Ne`` and ``eigenvector.load(key)[t, e]`` -> [Lz, Ly, Lx, Nc].
+
+The classes here provide that contract for numpy ``.npy`` files (read through
+a memory map, so only the slices asked for are read) and for arrays already
+in memory.  ``ElementalNpy`` reads elementals in the reference's file layout
+[D, P, Lt, Ne, Ne] (lattice/preset.py:173-181) and ``save_elemental`` writes it.
+"""
+
+from __future__ import annotations
+
+from typing import List, Sequence
+
+import numpy as np
+
+
+class FileMetaData:
+    def __init__(self, shape: List[int], dtype: str = "<c16", extra=None):
+        self.shape = shape
+        self.dtype = dtype
+        self.extra = extra
+
+
+class ArrayFileData:
+    """``FileData`` over a numpy array or read-only memory map."""
+
+    def __init__(self, array: np.ndarray, file: str | None = None):
+        self._a = array
+        self.file = file
+        self.shape = list(array.shape)
+        self.dtype = array.dtype.str
+        self.time_in_sec = 0.0
+        self.size_in_byte = 0
+
+    def __getitem__(self, key):
+        out = np.array(self._a[key], copy=True)
+        self.size_in_byte += out.nbytes
+        return out
+
+
+class _NpyFile:
+    def __init__(self, prefix: str, suffix: str, shape: Sequence[int] | None):
+        self.prefix = prefix
+        self.suffix = suffix
+        self.shape = None if shape is None else list(shape)
+        self._cache_name = None
+        self._cache = None
+
+    def _open(self, key: str) -> ArrayFileData:
+        name = f"{self.prefix}{key}{self.suffix}"
+        if name != self._cache_name:
+            arr = np.load(name, mmap_mode="r", allow_pickle=False)
+            if self.shape is not None and list(arr.shape) != self.shape:
+                raise ValueError(f"{name}: shape {list(arr.shape)} does not match {self.shape}")
+            self._cache_name, self._cache = name, ArrayFileData(arr, name)
+        return self._cache
+
+
+class GaugeField:
+    pass
+
+
+class Eigenvector:
+    def __init__(self, Ne: int):
+        self.Ne = Ne
+
+
+class GaugeFieldNpy(_NpyFile, GaugeField):
+    """Gauge configuration stored as .npy with shape [Lt, Lz, Ly, Lx, Nd, Nc, Nc]."""
+
+    def __init__(self, prefix: str, suffix: str = ".npy", shape: Sequence[int] | None = None):
+        _NpyFile.__init__(self, prefix, suffix, shape)
+
+    def load(self, key: str) -> ArrayFileData:
+        return self._open(key)
+
+
+class GaugeFieldHostmem(GaugeField):
+    """Gauge configuration held in memory, [Lt, Lz, Ly, Lx, Nd, Nc, Nc]."""
+
+    def __init__(self, array: np.ndarray):
+        self.data = ArrayFileData(np.asarray(array))
+
+    def load(self, key: str | None = None) -> ArrayFileData:
+        return self.data
+
+
+class EigenvectorNpy(_NpyFile, Eigenvector):
+    """Eigenvectors stored as .npy [Lt, totNe, Lz, Ly, Lx, Nc] (lattice/preset.py:66-74)."""
+
+    def __init__(self, prefix: str, suffix: str = ".npy", shape: Sequence[int] | None = None, totNe: int = 70):
+        _NpyFile.__init__(self, prefix, suffix, shape)
+        Eigenvector.__init__(self, totNe)
+
+    def load(self, key: str) -> ArrayFileData:
+        return self._open(key)
+
+
+class EigenvectorHostmem(Eigenvector):
+    """Eigenvectors held in memory, [Lt, Ne, Lz, Ly, Lx, Nc] (lattice/preset.py:77-89)."""
+
+    def __init__(self, host_ndarray: np.ndarray, totNe: int | None = None):
+        arr = np.asarray(host_ndarray)
+        Eigenvector.__init__(self, arr.shape[1] if totNe is None else totNe)
+        self.data = ArrayFileData(arr)
+
+    def load(self, key: str | None = None) -> ArrayFileData:
+        return self.data
+
+
+class ElementalNpy(_NpyFile):
+    """Elementals in the reference's file layout [D, P, Lt, Ne, Ne] (lattice/preset.py:173-181)."""
+
+    def __init__(self, prefix: str, suffix: str, shape: Sequence[int] | None = None, totNe: int = 70):
+        _NpyFile.__init__(self, prefix, suffix, shape)
+        self.Ne = totNe
+
+    def load(self, key: str) -> ArrayFileData:
+        return self._open(key)
+
+
+def save_elemental(path: str, data_t_major: np.ndarray) -> None:
+    """Write [Lt, D, P, Ne, Ne] as the file layout [D, P, Lt, Ne, Ne] (tests/test_elemental.py:48)."""
+    np.save(path, np.ascontiguousarray(np.asarray(data_t_major).transpose(1, 2, 0, 3, 4)))

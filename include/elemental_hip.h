@@ -1,0 +1,153 @@
+/*
+ * elemental_hip.h -- C ABI of libelemental_hip.so, the MI355X (gfx950)
+ * elemental-generation engine.
+ *
+ * The reference has no FFI; its drop-in seam for this path is Python: the
+ * backend module object (lattice/backend.py:7-33) and the ElementalGenerator
+ * contract (lattice/generator/elemental.py:17-26 constructor, :102-105 load,
+ * :290-338 calc).  Each entry point below replaces one piece of that
+ * contract; the Python class easydistillation_amd.ElementalGenerator binds
+ * them through ctypes.
+ *
+ * Conventions
+ *   - every function returns int: ED_OK (0) or a negative ED_ERR_* code;
+ *     ed_last_error(ctx) gives the message of the last failure on ctx.
+ *   - no C++ exception crosses this boundary.
+ *   - a context binds one HIP device and one stream; it is not re-entrant
+ *     (one context per host thread).  Calls on one context are ordered on its
+ *     stream.  Calls on two contexts that share a device run independently.
+ *   - caller-owned pointers are only read during the call; the library copies
+ *     what it keeps (gauge links, eigenvectors, weights).
+ *   - complex arrays are interleaved (re, im); ED_C128 = complex<double>,
+ *     ED_C64 = complex<float>.
+ */
+#ifndef ELEMENTAL_HIP_H
+#define ELEMENTAL_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ed_context ed_context;
+
+enum ed_status {
+  ED_OK = 0,
+  ED_ERR_ARG = -1,    /* invalid argument (shape, dtype, range, null pointer)      */
+  ED_ERR_STATE = -2,  /* call order: lattice / gauge / eigvecs / momenta not set   */
+  ED_ERR_HIP = -3,    /* a HIP runtime call failed                                 */
+  ED_ERR_ALLOC = -4,  /* device or host allocation failed                          */
+  ED_ERR_UNSUPPORTED = -5
+};
+
+enum ed_dtype { ED_C128 = 0, ED_C64 = 1 };
+
+enum ed_mode {
+  ED_MODE_FP64 = 0,  /* complex128 stencil and contraction (reference numerics)          */
+  ED_MODE_MIXED = 1  /* complex64 stencil, FP32-MFMA contraction, FP64 phase table and  */
+                     /* FP64 cross-chunk spatial reduction                              */
+};
+
+/* ed_create / ed_destroy: one context per device and host thread.
+ * Replaces the per-instance state of ElementalGenerator.__init__
+ * (lattice/generator/elemental.py:17-59).  stream: the context creates its
+ * own stream (default priority) unless ed_set_stream is called. */
+int ed_create(int device, ed_context** ctx);
+int ed_destroy(ed_context* ctx);
+const char* ed_last_error(const ed_context* ctx);
+
+/* Use an external stream (e.g. torch.cuda.current_stream().cuda_stream);
+ * NULL restores the context's own stream. */
+int ed_set_stream(ed_context* ctx, void* hip_stream);
+
+/* latt_size = [Lx, Ly, Lz, Lt] and Ne = eigenvector.Ne
+ * (elemental.py:30,53; lattice/preset.py:16-19). */
+int ed_set_lattice(ed_context* ctx, int Lx, int Ly, int Lz, int Lt, int Ne);
+
+/* Spatial links U[3][Lt][Lz][Ly][Lx][3][3] (mu = x, y, z), row-major,
+ * U[..][a][b] acting on colour b -- the array elemental.py:103 builds from
+ * gauge_field.load(key)[:].transpose(4,0,1,2,3,5,6)[:3].
+ * dtype ED_C128 or ED_C64; on_device != 0 means U is a device pointer. */
+int ed_set_gauge(ed_context* ctx, const void* U, int dtype, int on_device);
+
+/* Eigenvectors V[Lt][Ne][Lz][Ly][Lx][3] (eigenvector.load(key)[t, e],
+ * elemental.py:297-298).  Stored as complex64: complex128 input is rounded
+ * to complex64 exactly as the reference's c8 staging buffer does
+ * (elemental.py:55). */
+int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device);
+
+/* Momentum list n[P][3] = (nx, ny, nz); output index p follows this order
+ * and Phi_n(x) = exp(+2 pi i (nx x/Lx + ny y/Ly + nz z/Lz))
+ * (lattice/insertion/phase.py:11-13,41-46; elemental.py:322). */
+int ed_set_momenta(ed_context* ctx, const int32_t* n, int P);
+
+/* num_nabla in [0, 3]: D = (3^(num_nabla+1)-1)/2 derivative combinations in
+ * the index order of lattice/insertion/derivative.py:23-33
+ * (elemental.py:48-49). */
+int ed_set_derivatives(ed_context* ctx, int num_nabla);
+
+/* Blending weights coeff[Ne][Ne] (float64, host), multiplied elementwise into
+ * every E[d][p] (elemental.py:61-100,330-337); NULL disables blending. */
+int ed_set_weights(ed_context* ctx, const double* coeff);
+
+/* Elementals for timeslices t0 .. t0+nt-1:
+ *   out[nt][D][P][Ne][Ne] complex128,
+ *   E[d][p][e][f] = sum_splits (-1)^{#right} sum_{x,c} Phi_p(x)
+ *                   conj(W_left[e](x)_c) W_right[f](x)_c
+ * (elemental.py:290-338; calc(t) returns out[t - t0]).
+ * mode: ED_MODE_FP64 or ED_MODE_MIXED.  out_on_device != 0: out is a device
+ * pointer written on the context stream (the call returns after enqueueing);
+ * otherwise out is host memory and the call returns when it is filled. */
+int ed_compute(ed_context* ctx, int t0, int nt, int mode, void* out, int out_on_device);
+
+/* Block until all work enqueued on the context stream is done. */
+int ed_synchronize(ed_context* ctx);
+
+/* Integer tuning / testing knobs (no counterpart in the reference):
+ *   ED_OPT_HERMITIAN      1 (default): derive E[rev d] from E[d] through
+ *                         E_{rev d}[p] = (-1)^{|d|} E_d[-p]^dagger; 0: compute all d.
+ *   ED_OPT_TBATCH         timeslices per batch (0 = automatic).
+ *   ED_OPT_WS_MB          workspace budget in MiB for the automatic batch.
+ *   ED_OPT_PROFILE        1: record HIP events around every kernel launch. */
+enum ed_option { ED_OPT_HERMITIAN = 1, ED_OPT_TBATCH = 2, ED_OPT_WS_MB = 3, ED_OPT_PROFILE = 4 };
+int ed_set_option(ed_context* ctx, int option, int64_t value);
+
+/* Kernel timing collected while ED_OPT_PROFILE = 1, per kernel family:
+ *   ED_KERNEL_STENCIL  covariant hops (one launch per derivative level),
+ *   ED_KERNEL_CONTRACT the two-stage contraction,
+ *   ED_KERNEL_STAGE    staging of V into the workspace.
+ * total_ms is the sum of HIP-event durations on the launch stream and
+ * launches their number; both are reset by ed_profile_reset. */
+enum ed_kernel { ED_KERNEL_STENCIL = 0, ED_KERNEL_CONTRACT = 1, ED_KERNEL_STAGE = 2 };
+int ed_profile_read(ed_context* ctx, int kernel, double* total_ms, int64_t* launches);
+int ed_profile_reset(ed_context* ctx);
+
+/* Plan figures for the current lattice / derivative / momentum settings
+ * (used for roofline accounting; no counterpart in the reference):
+ *   ED_Q_CONTRACT_FLOPS_PER_T  MFMA flops the contraction executes per timeslice
+ *   ED_Q_ALGO_FLOPS_PER_T      8 Ne^2 (3 V3) S P, the direct split-by-split
+ *                              formulation (S = number of split terms)
+ *   ED_Q_PRIMARY_DERIVATIVES   derivatives contracted (the others follow by
+ *                              Hermiticity)
+ *   ED_Q_MOMENTUM_BLOCKS       blocks of 16 base momenta
+ *   ED_Q_HOPS_PER_T            covariant hops per timeslice (D - 1)
+ *   ED_Q_SPLITS                split terms over all derivatives (S) */
+enum ed_query {
+  ED_Q_CONTRACT_FLOPS_PER_T = 1,
+  ED_Q_ALGO_FLOPS_PER_T = 2,
+  ED_Q_PRIMARY_DERIVATIVES = 3,
+  ED_Q_MOMENTUM_BLOCKS = 4,
+  ED_Q_HOPS_PER_T = 5,
+  ED_Q_SPLITS = 6
+};
+int ed_query(ed_context* ctx, int what, int64_t* value);
+
+/* Library version string (static storage). */
+const char* ed_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ELEMENTAL_HIP_H */

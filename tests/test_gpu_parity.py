@@ -1,0 +1,220 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle and fixtures.
+
+Bars: FP64 form <= 1e-12 and mixed form <= 1e-6, both as the normwise error
+max|dE| / max|E| of every (d, p) block (elementwise relative error is
+ill-conditioned on near-zero elementals).
+"""
+
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+import easydistillation_amd as ed
+from easydistillation_amd import _native
+from easydistillation_amd import synthetic as S
+from oracle import elemental_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def blockwise(E, Eref):
+    """max over (d, p) blocks of max|dE| / max|Eref| for arrays [t, D, P, Ne, Ne].
+
+    A block whose reference maximum is below 1e-8 of the largest block of the
+    same derivative vanishes by symmetry (e.g. a Nyquist momentum component
+    along the derivative's direction); its relative error is 0/0, so it is
+    measured against that derivative's largest block instead.
+    """
+    E = np.asarray(E)
+    Eref = np.asarray(Eref)
+    d = np.abs(E - Eref)
+    # move t into the block: [t, D, P, Ne, Ne] -> [D, P, t*Ne*Ne]
+    d = d.transpose(1, 2, 0, 3, 4).reshape(d.shape[1], d.shape[2], -1).max(-1)
+    r = np.abs(Eref).transpose(1, 2, 0, 3, 4).reshape(d.shape[0], d.shape[1], -1).max(-1)
+    row = r.max(1, keepdims=True)
+    den = np.where(r >= 1e-8 * row, r, row)
+    return float((d / np.maximum(den, 1e-300)).max())
+
+
+def run(latt, Ne, num_nabla, moms, U, V, mode="fp64", hermitian=1, tbatch=0, dilution=None):
+    g = ed.ElementalGenerator(
+        latt, None, None, num_nabla, moms, dilution=dilution, is_blending=dilution is not None, Ne=Ne, mode=mode
+    )
+    g.set_option(_native.ED_OPT_HERMITIAN, hermitian)
+    if tbatch:
+        g.set_option(_native.ED_OPT_TBATCH, tbatch)
+    g.load_arrays(U, V)
+    return g.calc_all()
+
+
+def load_fixture(golden_dir, name):
+    z = np.load(f"{golden_dir}/{name}.npz")
+    latt = [int(v) for v in z["latt"]]
+    moms = [tuple(int(c) for c in m) for m in z["momenta"]]
+    return latt, int(z["num_nabla"]), moms, z["U"], z["V"], z["E"].transpose(2, 0, 1, 3, 4)
+
+
+@pytest.mark.parametrize("name", ["config_a", "ref_shape", "brute_force"])
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_fixtures(golden_dir, name, mode):
+    latt, nn, moms, U, V, Eref = load_fixture(golden_dir, name)
+    say(f"fixture {name} mode={mode} latt={latt} Ne={V.shape[1]} nabla={nn} P={len(moms)}")
+    E = run(latt, V.shape[1], nn, moms, U, V, mode)
+    err = blockwise(E, Eref)
+    say(f"  err={err:.3e}")
+    assert err <= (FP64_TOL if mode == "fp64" else MIXED_TOL)
+
+
+def test_hermitian_shortcut_is_exact(golden_dir):
+    latt, nn, moms, U, V, Eref = load_fixture(golden_dir, "ref_shape")
+    say("hermitian on/off")
+    E1 = run(latt, V.shape[1], nn, moms, U, V, hermitian=1)
+    E0 = run(latt, V.shape[1], nn, moms, U, V, hermitian=0)
+    assert blockwise(E0, Eref) <= FP64_TOL
+    assert blockwise(E1, E0) <= 1e-13
+
+
+CASES = [
+    # (latt, Ne, num_nabla, momenta, links)
+    ([3, 5, 7, 2], 37, 2, S.momenta_by_norm(2), "haar"),
+    ([6, 4, 2, 1], 1, 1, S.unit_momenta(), "haar"),
+    ([5, 5, 5, 3], 16, 0, S.momenta_by_norm(3), "smooth"),
+    ([4, 3, 3, 1], 7, 3, [(0, 0, 0), (1, 0, 0), (0, 0, -1)], "haar"),
+    ([4, 4, 6, 2], 20, 2, S.momenta_by_norm(5), "haar"),  # 57 momenta: two momentum blocks
+    ([4, 4, 4, 2], 17, 1, [(1, 0, 0), (1, 0, 0), (0, 1, 1), (-1, 0, 0)], "haar"),  # repeats, no p = 0
+    ([4, 4, 4, 1], 9, 2, [(0, 0, 0), (0, 0, 0), (0, 2, 0)], "smooth"),  # p = 0 twice
+    ([8, 8, 8, 2], 40, 2, S.momenta_by_norm(4), "haar"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_uneven_shapes_against_oracle(case):
+    latt, Ne, nn, moms, kind = CASES[case]
+    say(f"case {case}: latt={latt} Ne={Ne} nabla={nn} P={len(moms)} links={kind}")
+    U, V = S.make_inputs_np(latt, Ne, kind, seed=100 + case)
+    Eref = O.elemental(V, U, latt, nn, moms)
+    E = run(latt, Ne, nn, moms, U, V, "fp64")
+    err = blockwise(E, Eref)
+    say(f"  fp64 err={err:.3e}")
+    assert err <= FP64_TOL
+    Em = run(latt, Ne, nn, moms, U, V, "mixed")
+    errm = blockwise(Em, Eref)
+    say(f"  mixed err={errm:.3e}")
+    assert errm <= MIXED_TOL
+
+
+def test_blending_against_oracle():
+    latt, Ne = [4, 4, 4, 2], 10
+    dil = ([8, 12], [4, 6])
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=21)
+    moms = S.unit_momenta()
+    W = O.blending_coeff(Ne, dil)
+    Eref = O.elemental(V, U, latt, 2, moms, stocastic_coeff=W)
+    say("blending")
+    assert blockwise(run(latt, Ne, 2, moms, U, V, dilution=dil), Eref) <= FP64_TOL
+    assert blockwise(run(latt, Ne, 2, moms, U, V, dilution=dil, hermitian=0), Eref) <= FP64_TOL
+
+
+def test_calc_paths_agree():
+    latt, Ne = [4, 6, 4, 5], 24
+    U, V = S.make_inputs_np(latt, Ne, "smooth", seed=5)
+    moms = S.momenta_by_norm(2)
+    say("calc paths")
+    g = ed.ElementalGenerator(latt, None, None, 2, moms, Ne=Ne)
+    g.load_arrays(U, V)
+    single = np.stack([g.calc(t) for t in range(latt[3])])
+    g.set_option(_native.ED_OPT_TBATCH, 2)
+    batched = g.calc_all()
+    again = g.calc(3)
+    assert np.array_equal(single, batched)
+    assert np.array_equal(again, batched[3])
+    Eref = O.elemental(V, U, latt, 2, moms)
+    assert blockwise(batched, Eref) <= FP64_TOL
+
+
+def test_load_from_containers(tmp_path):
+    latt = [4, 4, 4, 3]
+    Lx, Ly, Lz, Lt = latt
+    U, V = S.make_inputs_np(latt, 12, "haar", seed=31)
+    full = np.zeros((Lt, Lz, Ly, Lx, 4, 3, 3), complex)
+    full[..., :3, :, :] = U.transpose(1, 2, 3, 4, 0, 5, 6)
+    full[..., 3, :, :] = np.eye(3)
+    np.save(tmp_path / "c0.gauge.npy", full)
+    Vtot = np.concatenate([V, V[:, :2]], axis=1).astype(np.complex128)  # file holds more vectors than used
+    np.save(tmp_path / "c0.evec.npy", Vtot)
+    gauge = ed.GaugeFieldNpy(f"{tmp_path}/", ".gauge.npy", list(full.shape))
+    evec = ed.EigenvectorNpy(f"{tmp_path}/", ".evec.npy", list(Vtot.shape), 12)
+    moms = S.unit_momenta()
+    say("containers")
+    g = ed.ElementalGenerator(latt, gauge, evec, 1, moms)
+    g.load("c0")
+    E = np.stack([g.calc(t) for t in range(Lt)])
+    assert blockwise(E, O.elemental(V, U, latt, 1, moms)) <= FP64_TOL
+
+
+def test_device_inputs_and_outputs():
+    import torch
+
+    latt, Ne = [6, 4, 4, 2], 20
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=41)
+    moms = S.momenta_by_norm(2)
+    say("device tensors")
+    ed.set_backend("torch")
+    try:
+        g = ed.ElementalGenerator(latt, None, None, 2, moms, Ne=Ne)
+        g.load_arrays(torch.from_numpy(U).cuda(), torch.from_numpy(V).cuda())
+        E = g.calc_all()
+        assert E.is_cuda and E.dtype == torch.complex128
+        E = E.cpu().numpy()
+    finally:
+        ed.set_backend("hip")
+    assert blockwise(E, O.elemental(V, U, latt, 2, moms)) <= FP64_TOL
+
+
+def test_config_b_shape_properties():
+    """16^3, Ne = 100, 33 momenta, 13 derivatives: invariants at the real per-t shape."""
+    import torch
+
+    latt, Ne = [16, 16, 16, 2], 100
+    moms = S.momenta_by_norm(4)
+    say("config B shape: inputs")
+    U = S.gauge_torch(latt, "smooth", 0.3, seed=3, device="cuda")
+    V = S.eigvecs_torch(latt, Ne, seed=4, device="cuda")
+    g = ed.ElementalGenerator(latt, None, None, 2, moms, Ne=Ne)
+    g.load_arrays(U, V)
+    say("config B shape: fp64")
+    E = g.calc_all()
+    say("config B shape: mixed")
+    Em = g.calc_all(mode="mixed")
+    idx = {m: i for i, m in enumerate(moms)}
+    # V^dag V = 1 to complex64 rounding
+    assert np.abs(E[:, 0, idx[(0, 0, 0)]] - np.eye(Ne)).max() < 1e-5
+    # E_{rev d}[p] = (-1)^{|d|} E_d[-p]^dagger
+    dl = O.derivative_list(2)
+    for di, d in enumerate(dl):
+        rd = dl.index(tuple(reversed(d)))
+        for m in [(0, 0, 0), (1, 0, 0), (1, 1, -1), (0, 2, 0)]:
+            mm = tuple(-c for c in m)
+            lhs = E[:, rd, idx[m]]
+            rhs = (-1) ** len(d) * np.conj(np.swapaxes(E[:, di, idx[mm]], -1, -2))
+            assert np.abs(lhs - rhs).max() <= 1e-12 * np.abs(lhs).max()
+    err = blockwise(Em, E)
+    say(f"  mixed vs fp64 err={err:.3e}")
+    assert err <= MIXED_TOL
+    # one timeslice against the oracle
+    say("config B shape: oracle t=0")
+    Uh, Vh = U.cpu().numpy(), V.cpu().numpy()
+    sub = [(0, 0, 0), (1, 0, 0), (0, -1, 1), (-1, 1, -1), (0, 0, -2)]
+    Eref0 = O.elemental_timeslice(Vh[0], Uh[:, 0], latt, 2, sub)
+    got = E[:1][:, :, [idx[m] for m in sub]]
+    assert blockwise(got, Eref0[None]) <= FP64_TOL

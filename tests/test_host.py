@@ -1,0 +1,125 @@
+"""Host logic and C-ABI surface, on the CPU (no GPU compute calls)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import easydistillation_amd as ed
+from easydistillation_amd import _native
+from oracle import elemental_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "elemental_hip.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ed_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _native.load_library()
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_native.SIGNATURES)
+    assert lib.ed_version().startswith(b"elemental_hip")
+
+
+def test_no_silent_cpu_fallback():
+    try:
+        import torch
+
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(_native.NativeError):
+        _native.Context(0)
+    with pytest.raises(_native.NativeError):
+        ed.ElementalGenerator([4, 4, 4, 2], None, None, 1, [(0, 0, 0)], Ne=4)
+
+
+def test_c_abi_argument_errors_without_context():
+    lib = _native.load_library()
+    assert lib.ed_create(0, None) == -1
+    assert lib.ed_destroy(None) == -1
+    assert lib.ed_compute(None, 0, 1, 0, None, 0) == -1
+    assert lib.ed_last_error(None) == b"null context"
+
+
+def test_derivative_map_matches_oracle():
+    for n in range(40):
+        assert ed.derivative(n) == O.derivative(n)
+        assert ed.derivative_index(ed.derivative(n)) == n
+    assert [ed.num_derivative(k) for k in range(4)] == [O.num_derivative(k) for k in range(4)]
+
+
+def test_blending_weights_match_oracle():
+    for Ne, dil in [(5, ([4, 6], [2, 3])), (6, ([10, 10], 3)), (4, ([8], [4])), (7, ([3, 9, 5], [2, 3, 2]))]:
+        assert np.array_equal(ed.blending_weights(Ne, dil), O.blending_coeff(Ne, dil))
+    with pytest.raises(AssertionError):
+        ed.blending_weights(5, ([4, 6], [2, 2]))
+
+
+def test_momentum_phase_matches_oracle():
+    mp = ed.MomentumPhase([4, 5, 3, 8])
+    for n in [(0, 0, 0), (1, 0, 0), (0, -2, 1), (3, 7, -5)]:
+        assert np.abs(mp.get(n) - O.momentum_phase([4, 5, 3, 8], n)).max() < 1e-13
+
+
+def test_backend_rejects_cpu_backends():
+    with pytest.raises(ValueError):
+        ed.set_backend("numpy")
+    ed.set_backend("torch")
+    assert ed.get_backend() == "torch"
+    ed.set_backend("hip")
+    assert ed.get_backend() == "hip"
+
+
+def test_containers_follow_load_contract(tmp_path):
+    V = (np.arange(2 * 3 * 2 * 2 * 2 * 3) * (1 + 1j)).reshape(2, 3, 2, 2, 2, 3).astype(np.complex128)
+    np.save(tmp_path / "cfg.evec.npy", V)
+    ev = ed.EigenvectorNpy(f"{tmp_path}/", ".evec.npy", list(V.shape), 2)
+    data = ev.load("cfg")
+    assert ev.Ne == 2
+    assert np.array_equal(data[1, 0], V[1, 0])
+    assert np.array_equal(data[0, :2], V[0, :2])
+    Ug = np.zeros((2, 2, 2, 2, 4, 3, 3), complex)
+    np.save(tmp_path / "cfg.gauge.npy", Ug)
+    g = ed.GaugeFieldNpy(f"{tmp_path}/", ".gauge.npy")
+    assert g.load("cfg")[:].shape == Ug.shape
+    with pytest.raises(ValueError):
+        ed.EigenvectorNpy(f"{tmp_path}/", ".evec.npy", [9, 9], 2).load("cfg")
+    E = np.ones((2, 4, 3, 5, 5), complex)
+    ed.save_elemental(tmp_path / "cfg.elem.npy", E)
+    back = ed.ElementalNpy(f"{tmp_path}/", ".elem.npy", [4, 3, 2, 5, 5], 5).load("cfg")[:]
+    assert back.shape == (4, 3, 2, 5, 5)
+
+
+def test_momentum_sets():
+    from easydistillation_amd import synthetic as S
+
+    assert len(S.momenta_by_norm(2)) == 19
+    assert len(S.momenta_by_norm(3)) == 27
+    assert len(S.momenta_by_norm(4)) == 33
+    m = S.momenta_by_norm(4)
+    assert m[0] == (0, 0, 0) and set(m) == {tuple(-c for c in n) for n in m}
+
+
+def test_synthetic_inputs_are_valid():
+    from easydistillation_amd import synthetic as S
+
+    for kind in ("haar", "smooth"):
+        U, V = S.make_inputs_np([3, 4, 2, 2], 6, kind, seed=3)
+        u = U.reshape(-1, 3, 3)
+        assert np.abs(u @ u.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-13
+        assert np.abs(np.linalg.det(u) - 1).max() < 1e-13
+        assert V.dtype == np.complex64
+        G = np.einsum("tek,tfk->tef", V.reshape(2, 6, -1).conj().astype(complex), V.reshape(2, 6, -1).astype(complex))
+        assert np.abs(G - np.eye(6)).max() < 1e-6
