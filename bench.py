@@ -133,6 +133,13 @@ def main():
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument(
+        "--calibrate-copy",
+        type=int,
+        default=0,
+        metavar="MIB",
+        help="first run one device-to-device copy of this many MiB (known HBM bytes for PMC byte-counter scaling)",
+    )
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -165,6 +172,16 @@ def main():
     V = S.eigvecs_torch(latt, Ne, seed=12, device=dev, log=log)
     torch.cuda.synchronize()
     log(f"inputs built in {time.time() - t0:.1f} s")
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n} bytes read + {n} bytes written")
+        del src, dst
 
     forms = [f for f in args.forms.split(",") if f]
     results = {}

@@ -126,6 +126,7 @@ struct ed_context {
   int nd = 0, npb = 1, nbase = 0, zero_idx = -1;
   int64_t mfma_per_chunk_stage1 = 0;  // sum over primary d of 16 * 4 * nks(d)
   int64_t total_splits = 0;           // split terms of elemental.py:309-321 over all d
+  int nks_max = 1;                    // largest stage-1 k-step count over primary d
   DevBuf<DPlan> dplan;
   DevBuf<SplitTerm> splits;
   DevBuf<int> pidx;  // plus [npb*16] then minus [npb*16]
@@ -229,6 +230,7 @@ void build_plan(ed_context* c) {
   std::vector<SplitTerm> sp;
   c->mfma_per_chunk_stage1 = 0;
   c->total_splits = 0;
+  c->nks_max = 1;
   for (int d = 0; d < D; ++d) {
     std::vector<int> t = derivative_tuple(d);
     c->total_splits += (int64_t)1 << t.size();
@@ -267,6 +269,7 @@ void build_plan(ed_context* c) {
     }
     p.nsplit = (int)sp.size() - p.first;
     c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4 * ((3 * p.nsplit + 3) / 4);
+    c->nks_max = std::max(c->nks_max, (3 * p.nsplit + 3) / 4);
     dpl.push_back(p);
   }
   c->nd = (int)dpl.size();
@@ -409,17 +412,16 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     const int nb = std::min(TB, t0 + nt - tb0);
     // V(t) -> W slot 0
     c->launch(ED_KERNEL_STAGE, [&] {
-      dim3 grid((3 * c->V3 + 255) / 256, c->Ne, nb);
+      dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
       ed::stage_v_kernel<T><<<grid, 256, 0, c->stream>>>(c->V.p, W, g, tb0);
     });
     // covariant hops level by level
-    const int echunk = 8, nech = (c->Ne + echunk - 1) / echunk;
     for (int k = 1; k <= c->num_nabla; ++k) {
       const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
       const int child0 = level_start(k);
       c->launch(ED_KERNEL_STENCIL, [&] {
-        dim3 grid((c->V3 + 255) / 256, 3 * nparent, nb * nech);
-        ed::hop_kernel<T><<<grid, 256, 0, c->stream>>>(W, U, g, tb0, parent0, child0, echunk, nech);
+        dim3 grid((c->V3 + 3) / 4, 3 * nparent, nb);
+        ed::hop_kernel<T><<<grid, 256, 0, c->stream>>>(W, U, g, tb0, parent0, child0);
       });
     }
     // contraction
@@ -443,7 +445,16 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     a.psum = c->psum.p;
     const int nblk = (a.nwork + 7) / 8 * 8;
     c->launch(ED_KERNEL_CONTRACT, [&] {
-      ed::contract_kernel<T><<<nblk, 256, 0, c->stream>>>(W, ph, ph + phn, out, a);
+      if (c->nks_max <= 1)
+        ed::contract_kernel<T, 1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(W, ph, ph + phn, out, a);
+      else if (c->nks_max <= 2)
+        ed::contract_kernel<T, 2><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(W, ph, ph + phn, out, a);
+      else if (c->nks_max <= 3)
+        ed::contract_kernel<T, 3><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(W, ph, ph + phn, out, a);
+      else if (c->nks_max <= 6)
+        ed::contract_kernel<T, 6><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(W, ph, ph + phn, out, a);
+      else
+        fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", 4 * c->nks_max / 3);
     });
     if (!out_dev) {
       HIPCHK(hipMemcpyAsync((double2*)out_host + (size_t)(tb0 - t0) * out_t, out, (size_t)nb * out_t * sizeof(double2),

@@ -1,11 +1,14 @@
 // ed_kernels.hpp -- gfx950 device code for elemental generation.
 //
 // Data layout in HBM (one timeslice batch of TB slices):
-//   W  [TB][nW][NeP][V3P][3]  complex<T>, nW = D displaced vectors
+//   W  [TB][nW][V3P][3][NeP]  complex<T>, nW = D displaced vectors
 //      W_n = nabla_{d_k} ... nabla_{d_1} V for derivative(n) = (d_1..d_k)
 //      (lattice/generator/elemental.py:279-288 applies tuple elements in
-//      order); NeP = Ne rounded up to 16, V3P = V3 rounded up to 16; padded
-//      rows and sites hold zeros so the contraction needs no tail masks.
+//      order).  Site-major with the eigenvector index fastest: the 16
+//      eigenvectors of a contraction tile at one (site, colour) are one
+//      contiguous segment, and the hop kernel maps lanes to eigenvectors.
+//      NeP = Ne rounded up to 16, V3P = V3 rounded up to 16; padded rows and
+//      sites hold zeros so the contraction needs no tail masks.
 //   U  [3][Lt][V3][3][3]      complex<double> spatial links (elemental.py:103)
 //   Vd [Lt][Ne][V3][3]        complex<float>: the reference stages V in a
 //                             complex64 buffer (elemental.py:55)
@@ -63,18 +66,40 @@ struct Geom {
   int nW;
 };
 
+// element (n, x, c, e) of one timeslice's W block
+__device__ __forceinline__ size_t widx(const Geom& g, int n, int x, int c, int e) {
+  return (((size_t)n * g.V3P + x) * 3 + c) * g.NeP + e;
+}
+
 // --------------------------------------------------------------------------
-// Stage V(t) (complex64) into slot 0 of W in the compute precision.
-// grid (ceil(3 V3 / 256), Ne, TB)
+// Stage V(t) (complex64, [Ne][V3*3]) into slot 0 of W ([V3P*3][NeP]) in the
+// compute precision: a 64 x 64 tile transpose through LDS so that both the
+// read (along sites) and the write (along eigenvectors) are contiguous.
+// grid (ceil(3 V3 / 64), ceil(Ne / 64), TB), block 256
 // --------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void stage_v_kernel(const float2* __restrict__ V, c_t<T>* __restrict__ W,
                                                        Geom g, int tb0) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e = blockIdx.y, tb = blockIdx.z;
-  if (i >= 3 * g.V3) return;
-  const float2 v = V[((size_t)(tb0 + tb) * g.Ne + e) * (size_t)(3 * g.V3) + i];
-  W[(((size_t)tb * g.nW) * g.NeP + e) * (size_t)(3 * g.V3P) + i] = mk<T>((T)v.x, (T)v.y);
+  __shared__ float2 tile[64][65];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 64, e0 = blockIdx.y * 64, tb = blockIdx.z;
+  const int M = 3 * g.V3;
+  const float2* Vt = V + (size_t)(tb0 + tb) * g.Ne * (size_t)M;
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const int el = ty * 16 + k, e = e0 + el, m = m0 + tx;
+    tile[el][tx] = (e < g.Ne && m < M) ? Vt[(size_t)e * M + m] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;  // slot 0
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const int ml = ty * 16 + k, m = m0 + ml, e = e0 + tx;
+    if (m < M && e < g.Ne) {
+      const float2 v = tile[tx][ml];
+      Wt[(size_t)m * g.NeP + e] = mk<T>((T)v.x, (T)v.y);
+    }
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -82,20 +107,21 @@ __global__ __launch_bounds__(256) void stage_v_kernel(const float2* __restrict__
 //   out(x) = U_mu(x) in(x + mu) - U_mu(x - mu)^dag in(x - mu)
 // One launch computes every child of one derivative level: blockIdx.y
 // enumerates (parent, mu); parent = parent0 + y / 3, child = child0 + y.
-// A thread owns one site, keeps both links in registers and loops over a
-// chunk of eigenvectors, so U is read once per chunk and V/W reads along x
-// are contiguous across the wave.  Arithmetic is FP64 for both forms; the
-// complex64 form rounds once, when the result is stored.
-// grid (ceil(V3 / 256), 3 * nparent, TB * nechunk)
+// A wave owns one site (so both links are wave-uniform loads) and its lanes
+// walk the eigenvectors, reading and writing contiguous segments.
+// Arithmetic is FP64 for both forms; the complex64 form rounds once, when
+// the result is stored.
+// grid (ceil(V3 / 4), 3 * nparent, TB), block 256 (4 sites)
 // --------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void hop_kernel(c_t<T>* __restrict__ W, const double2* __restrict__ U, Geom g,
-                                                   int tb0, int parent0, int child0, int echunk, int nechunk) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   int tb0, int parent0, int child0) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (s >= g.V3) return;
   const int y = blockIdx.y;
   const int parent = parent0 + y / 3, mu = y % 3, child = child0 + y;
-  const int tb = blockIdx.z / nechunk, ec = blockIdx.z % nechunk;
+  const int tb = blockIdx.z;
   const int x = s % g.Lx, yy = (s / g.Lx) % g.Ly, z = s / (g.Lx * g.Ly);
   int sf, sb;
   if (mu == 0) {
@@ -117,16 +143,13 @@ __global__ __launch_bounds__(256) void hop_kernel(c_t<T>* __restrict__ W, const 
     uf[k] = Um[(size_t)s * 9 + k];
     ub[k] = Um[(size_t)sb * 9 + k];
   }
-  const size_t row = (size_t)3 * g.V3P;
-  const c_t<T>* in = W + ((size_t)tb * g.nW + parent) * g.NeP * row;
-  c_t<T>* out = W + ((size_t)tb * g.nW + child) * g.NeP * row;
-  const int e_end = min(g.Ne, (ec + 1) * echunk);
-  for (int e = ec * echunk; e < e_end; ++e) {
-    const c_t<T>* ie = in + (size_t)e * row;
+  const c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;
+  c_t<T>* Wo = const_cast<c_t<T>*>(Wt);
+  for (int e = lane; e < g.Ne; e += 64) {
     double2 af[3], ab[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const c_t<T> vf = ie[(size_t)sf * 3 + b], vb = ie[(size_t)sb * 3 + b];
+      const c_t<T> vf = Wt[widx(g, parent, sf, b, e)], vb = Wt[widx(g, parent, sb, b, e)];
       af[b] = make_double2(vf.x, vf.y);
       ab[b] = make_double2(vb.x, vb.y);
     }
@@ -142,7 +165,7 @@ __global__ __launch_bounds__(256) void hop_kernel(c_t<T>* __restrict__ W, const 
         re -= v.x * ab[b].x + v.y * ab[b].y;
         im -= v.x * ab[b].y - v.y * ab[b].x;
       }
-      out[(size_t)e * row + (size_t)s * 3 + a] = mk<T>((T)re, (T)im);
+      Wo[widx(g, child, s, a, e)] = mk<T>((T)re, (T)im);
     }
   }
 }
@@ -182,7 +205,10 @@ struct ContractArgs {
   const double* psum;    // [npb*16] sum_x Phi_b(x): V3 when b = 0 mod L, else 0
 };
 
-constexpr int SC = 16;  // sites per chunk (4 per wave in stage 1, 4 k-steps in stage 2)
+constexpr int SC = 16;      // sites per chunk (4 k-steps of the stage-2 MFMA)
+constexpr int NWAVE = 8;    // waves per workgroup
+constexpr int SPW = SC / NWAVE;   // stage-1 sites per wave and chunk
+constexpr int NBW = 16 / NWAVE;   // stage-2 N-blocks (eigenvector rows e) per wave
 
 // Workgroup id -> work item such that each XCD (workgroups are dispatched
 // round-robin over 8 XCDs) receives a contiguous range of work items, so the
@@ -192,12 +218,23 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return (bid & 7) * per + (bid >> 3);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256, 1) void contract_kernel(const c_t<T>* __restrict__ W, const T* __restrict__ phr,
+// One workgroup (8 waves, two per SIMD) owns a 16 x 16 (e, f) tile, one
+// primary derivative, one block of 16 base momenta and one timeslice, and
+// sweeps all sites in chunks of 16:
+//   stage 1: wave w forms the Q tiles of sites SPW*w.. of the chunk on MFMA
+//            (K = split x colour, NKS k-steps) and stores them to LDS;
+//   stage 2: wave w contracts the 16 sites of the chunk for the NBW rows
+//            e = NBW*w.. (16-pair N-blocks) against the 16 momenta (M),
+//            K = sites.
+// The next chunk's stage-1 operands and phases are loaded into registers
+// while the current chunk is contracted; LDS is double-buffered so one
+// barrier per chunk suffices.
+template <typename T, int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* __restrict__ W, const T* __restrict__ phr,
                                                            const T* __restrict__ phi, double2* __restrict__ out,
                                                            ContractArgs a) {
   typedef typename mfma<T>::acc acc_t;
-  __shared__ c_t<T> Qs[SC][16][16];
+  __shared__ c_t<T> Qs[2][SC][16][16];
 
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
@@ -213,84 +250,118 @@ __global__ __launch_bounds__(256, 1) void contract_kernel(const c_t<T>* __restri
   const DPlan dp = a.dplan[di];
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
-  const size_t row = (size_t)3 * g.V3P;
-  const c_t<T>* Wt = W + (size_t)tb * g.nW * g.NeP * row;
+  const size_t site = (size_t)3 * g.NeP;  // elements per site
+  const c_t<T>* Wt = W + (size_t)tb * g.nW * g.V3P * site;
   const int J = 3 * dp.nsplit;
-  const int nks = (J + 3) >> 2;
+  const int nks = (J + 3) >> 2;  // <= NKS
 
-  acc_t A1[4], A2[4], A3[4], A4[4];
-  double D1[4][4], D2[4][4], D3[4][4], D4[4][4];  // FP64 cross-chunk sums (used when T = float)
-  T zr[4], zi[4];
-  double Zr[4], Zi[4];
+  // per k-step operand bases (site 0) and coefficients; j >= J reads a valid
+  // element with coefficient 0
+  const c_t<T>* Lb[NKS];
+  const c_t<T>* Rb[NKS];
+  T cf[NKS];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
+  for (int ks = 0; ks < NKS; ++ks) {
+    const int j = 4 * ks + g4;
+    if (ks < nks && j < J) {
+      const SplitTerm st = a.splits[dp.first + j / 3];
+      const int c = j % 3;
+      Lb[ks] = Wt + widx(g, st.nL, 0, c, E0 + l16);
+      Rb[ks] = Wt + widx(g, st.nR, 0, c, F0 + l16);
+      cf[ks] = (T)st.coeff;
+    } else {
+      Lb[ks] = Wt;
+      Rb[ks] = Wt;
+      cf[ks] = 0;
+    }
+  }
+
+  const T* PR = phr + (size_t)pb * g.V3P * 16;
+  const T* PI = phi + (size_t)pb * g.V3P * 16;
+
+  // prefetch registers
+  c_t<T> lv[NKS][SPW], rv[NKS][SPW];
+  T ph_r[4], ph_i[4];
+  auto load_ops = [&](int c0) {
+    const int xb = c0 + SPW * wave;
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      if (ks < nks) {
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) {
+          lv[ks][i] = Lb[ks][(size_t)(xb + i) * site];
+          rv[ks][i] = Rb[ks][(size_t)(xb + i) * site];
+        }
+      }
+    }
+  };
+  auto load_phase = [&](int c0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const size_t xo = (size_t)(c0 + 4 * k + g4) * 16 + l16;
+      ph_r[k] = PR[xo];
+      ph_i[k] = PI[xo];
+    }
+  };
+
+  acc_t A1[NBW], A2[NBW], A3[NBW], A4[NBW];
+  double D1[NBW][4], D2[NBW][4], D3[NBW][4], D4[NBW][4];  // FP64 cross-chunk sums (T = float)
+  T zr[NBW], zi[NBW];
+  double Zr[NBW], Zi[NBW];
+#pragma unroll
+  for (int b = 0; b < NBW; ++b) {
     A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
     zr[b] = zi[b] = 0;
     Zr[b] = Zi[b] = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
   }
-
-  const T* PR = phr + (size_t)pb * g.V3P * 16;
-  const T* PI = phi + (size_t)pb * g.V3P * 16;
-
   // FP32 form: Q_d[e,f,x] often carries a large site-independent part (for
   // d = 0, |V_e(x)|^2 >= 0).  Stage 2 works on Q - c with c the mean over the
   // first chunk, and adds c * sum_x Phi_b(x) back in FP64, so FP32 rounding
   // scales with the fluctuation of Q rather than with its mean.
-  T cr[4] = {0, 0, 0, 0}, ci[4] = {0, 0, 0, 0};
+  T cr[NBW], ci[NBW];
+#pragma unroll
+  for (int b = 0; b < NBW; ++b) cr[b] = ci[b] = 0;
 
-  for (int c0 = 0; c0 < g.V3P; c0 += SC) {
-    // ---- stage 1: Q_d[e, f, x] for the 4 sites of this wave ----
-    acc_t qr[4], qi[4];
+  load_ops(0);
+  load_phase(0);
+  int buf = 0;
+  for (int c0 = 0; c0 < g.V3P; c0 += SC, buf ^= 1) {
+    // ---- stage 1: Q_d[e, f, x] for the SPW sites of this wave ----
+    acc_t qr[SPW], qi[SPW];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) qr[i] = qi[i] = acc_t{0, 0, 0, 0};
-    const int xb = c0 + 4 * wave;
-    for (int ks = 0; ks < nks; ++ks) {
-      const int j = 4 * ks + g4;
-      T lr[4], li[4], rr[4], ri[4];
-      if (j < J) {
-        const SplitTerm st = a.splits[dp.first + j / 3];
-        const int c = j % 3;
-        const c_t<T>* Lp = Wt + ((size_t)st.nL * g.NeP + E0 + l16) * row + c;
-        const c_t<T>* Rp = Wt + ((size_t)st.nR * g.NeP + F0 + l16) * row + c;
-        const T cf = (T)st.coeff;
+    for (int i = 0; i < SPW; ++i) qr[i] = qi[i] = acc_t{0, 0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const c_t<T> lv = Lp[(size_t)(xb + i) * 3];
-          const c_t<T> rv = Rp[(size_t)(xb + i) * 3];
-          lr[i] = cf * lv.x;
-          li[i] = cf * lv.y;
-          rr[i] = rv.x;
-          ri[i] = rv.y;
+    for (int ks = 0; ks < NKS; ++ks) {
+      if (ks < nks) {
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) {
+          // conj(L) R = (lr rr + li ri) + i (lr ri - li rr)
+          const T lr = cf[ks] * lv[ks][i].x, li = cf[ks] * lv[ks][i].y;
+          qr[i] = mfma<T>::mac(lr, rv[ks][i].x, qr[i]);
+          qr[i] = mfma<T>::mac(li, rv[ks][i].y, qr[i]);
+          qi[i] = mfma<T>::mac(lr, rv[ks][i].y, qi[i]);
+          qi[i] = mfma<T>::mac(-li, rv[ks][i].x, qi[i]);
         }
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) lr[i] = li[i] = rr[i] = ri[i] = 0;
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // conj(L) R = (lr rr + li ri) + i (lr ri - li rr)
-        qr[i] = mfma<T>::mac(lr[i], rr[i], qr[i]);
-        qr[i] = mfma<T>::mac(li[i], ri[i], qr[i]);
-        qi[i] = mfma<T>::mac(lr[i], ri[i], qi[i]);
-        qi[i] = mfma<T>::mac(-li[i], rr[i], qi[i]);
       }
     }
+    // operands of the next chunk are in flight during stage 2
+    if (c0 + SC < g.V3P) load_ops(c0 + SC);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < SPW; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Qs[4 * wave + i][mfma<T>::row(g4, r)][l16] = mk<T>(qr[i][r], qi[i][r]);
+      for (int r = 0; r < 4; ++r) Qs[buf][SPW * wave + i][mfma<T>::row(g4, r)][l16] = mk<T>(qr[i][r], qi[i][r]);
     __syncthreads();
 
     if constexpr (sizeof(T) == 4) {
       if (c0 == 0) {
         const int n0 = min(SC, g.V3);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
+        for (int b = 0; b < NBW; ++b) {
           T sr = 0, si = 0;
           for (int xl = 0; xl < n0; ++xl) {
-            const c_t<T> q = Qs[xl][4 * wave + b][l16];
+            const c_t<T> q = Qs[buf][xl][NBW * wave + b][l16];
             sr += q.x;
             si += q.y;
           }
@@ -303,31 +374,29 @@ __global__ __launch_bounds__(256, 1) void contract_kernel(const c_t<T>* __restri
     // ---- stage 2: phase contraction over the 16 sites of the chunk ----
 #pragma unroll
     for (int k = 0; k < SC / 4; ++k) {
-      const int xl = 4 * k + g4;
-      const size_t xo = (size_t)(c0 + xl) * 16 + l16;
-      const T pr = PR[xo], pi = PI[xo];
-      const bool real_site = (c0 + xl) < g.V3;
+      const T pr = ph_r[k], pi = ph_i[k];
+      const bool real_site = (c0 + 4 * k + g4) < g.V3;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        c_t<T> q = Qs[xl][4 * wave + b][l16];
+      for (int b = 0; b < NBW; ++b) {
+        c_t<T> qq = Qs[buf][4 * k + g4][NBW * wave + b][l16];
         if constexpr (sizeof(T) == 4) {
-          q.x = real_site ? q.x - cr[b] : (T)0;
-          q.y = real_site ? q.y - ci[b] : (T)0;
+          qq.x = real_site ? qq.x - cr[b] : (T)0;
+          qq.y = real_site ? qq.y - ci[b] : (T)0;
         }
-        A1[b] = mfma<T>::mac(pr, q.x, A1[b]);
-        A2[b] = mfma<T>::mac(pi, q.y, A2[b]);
-        A3[b] = mfma<T>::mac(pi, q.x, A3[b]);
-        A4[b] = mfma<T>::mac(pr, q.y, A4[b]);
-        zr[b] += q.x;
-        zi[b] += q.y;
+        A1[b] = mfma<T>::mac(pr, qq.x, A1[b]);
+        A2[b] = mfma<T>::mac(pi, qq.y, A2[b]);
+        A3[b] = mfma<T>::mac(pi, qq.x, A3[b]);
+        A4[b] = mfma<T>::mac(pr, qq.y, A4[b]);
+        zr[b] += qq.x;
+        zi[b] += qq.y;
       }
     }
-    __syncthreads();
+    if (c0 + SC < g.V3P) load_phase(c0 + SC);
 
-    if (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4) {
       // FP32 partial sums of one chunk are moved into FP64 accumulators.
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
+      for (int b = 0; b < NBW; ++b) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           D1[b][r] += (double)A1[b][r];
@@ -342,9 +411,9 @@ __global__ __launch_bounds__(256, 1) void contract_kernel(const c_t<T>* __restri
       }
     }
   }
-  if (sizeof(T) == 8) {
+  if constexpr (sizeof(T) == 8) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NBW; ++b) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         D1[b][r] = (double)A1[b][r];
@@ -363,8 +432,8 @@ __global__ __launch_bounds__(256, 1) void contract_kernel(const c_t<T>* __restri
   const size_t tbase = (size_t)(a.t_out0 + tb) * a.D * P;
   const int f = F0 + l16;
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int e = E0 + 4 * wave + b;
+  for (int b = 0; b < NBW; ++b) {
+    const int e = E0 + NBW * wave + b;
     const bool ok = (e < Ne) && (f < Ne);
     const double wef = (ok && a.wgt) ? a.wgt[(size_t)e * Ne + f] : 1.0;
     const double wfe = (ok && a.wgt) ? a.wgt[(size_t)f * Ne + e] : 1.0;
