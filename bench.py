@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "elementals/s (t x p x d x Nev^2) mixed vs FP64 on 1 GPU at <=1e-6 rel err"
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s dense MFMA: FP64, FP32
+PEAK_FP16 = 2500.0  # TFLOP/s dense FP16 MFMA
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
@@ -209,6 +210,14 @@ def main():
         h_ms, h_n = g.context.profile_read(_native.ED_KERNEL_STENCIL)
         s_ms, s_n = g.context.profile_read(_native.ED_KERNEL_STAGE)
         ex_flops_t = g.context.query(_native.ED_Q_CONTRACT_FLOPS_PER_T)
+        f16_flops_t = 0
+        if form == "mixed" and Lx * Ly * Lz >= 4096:
+            # the mixed kernel runs stage 1 on FP32 MFMA and stage 2 on FP16 MFMA; its
+            # roofline time is fp32_flops / FP32 peak + fp16_flops / FP16 peak, quoted
+            # as FP32-equivalent flops
+            ex_flops_t = g.context.query(_native.ED_Q_MIXED_F32_FLOPS_PER_T)
+            f16_flops_t = g.context.query(_native.ED_Q_MIXED_F16_FLOPS_PER_T)
+        equiv_flops_t = ex_flops_t + f16_flops_t * PEAK["mixed"] / PEAK_FP16
         algo_flops_t = g.context.query(_native.ED_Q_ALGO_FLOPS_PER_T)
         hops_t = g.context.query(_native.ED_Q_HOPS_PER_T)
         s = 16 if form == "fp64" else 8
@@ -218,9 +227,11 @@ def main():
             "elementals_per_s": Lt * P * D * Ne * Ne / dt,
             "contract_ms_per_launch": c_ms / max(c_n, 1),
             "contract_launches_per_step": c_n // args.steps,
-            "contract_tflops_executed": ex_flops_t * Lt / (c_ms / args.steps * 1e-3) / 1e12,
+            "contract_tflops_executed": equiv_flops_t * Lt / (c_ms / args.steps * 1e-3) / 1e12,
+            "contract_fp16_tflops": f16_flops_t * Lt / (c_ms / args.steps * 1e-3) / 1e12,
             "contract_tflops_algorithmic": algo_flops_t * Lt / (c_ms / args.steps * 1e-3) / 1e12,
-            "executed_flops_per_launch": ex_flops_t * Lt * args.steps / max(c_n, 1),
+            "executed_flops_per_launch": equiv_flops_t * Lt * args.steps / max(c_n, 1),
+            "executed_fp16_flops_per_launch": f16_flops_t * Lt * args.steps / max(c_n, 1),
             "stencil_gbs": hop_bytes_t * Lt / (h_ms / args.steps * 1e-3) / 1e9 if h_ms > 0 else None,
             "stencil_ms_per_step": h_ms / args.steps,
             "stage_ms_per_step": s_ms / args.steps,
@@ -270,8 +281,13 @@ def main():
             "unit": "TFLOP/s",
             "frac": r["contract_tflops_executed"] / PEAK[head],
             "traffic": None,
-            "kernel": "contract_kernel<%s>" % ("float" if head == "mixed" else "double"),
-            "flops_basis": "executed MFMA flops of the factorized contraction per launch",
+            "kernel": "contract_mixed_f16_kernel" if head == "mixed" else "contract_kernel<double>",
+            "flops_basis": (
+                "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
+                "MFMA flops are converted to FP32-equivalent at the FP32:FP16 peak ratio 157.3:2500"
+                if head == "mixed"
+                else "executed FP64 MFMA flops of the factorized contraction per launch"
+            ),
             "executed_flops_per_launch": r["executed_flops_per_launch"],
             "avg_launch_ms": r["contract_ms_per_launch"],
             "algorithmic_equivalent_tflops": r["contract_tflops_algorithmic"],

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libelemental_hip.so")
 ED_OK = 0
 ED_C128, ED_C64 = 0, 1
 ED_MODE_FP64, ED_MODE_MIXED = 0, 1
-ED_OPT_HERMITIAN, ED_OPT_TBATCH, ED_OPT_WS_MB, ED_OPT_PROFILE = 1, 2, 3, 4
+ED_OPT_HERMITIAN, ED_OPT_TBATCH, ED_OPT_WS_MB, ED_OPT_PROFILE, ED_OPT_MIXED_STAGE2 = 1, 2, 3, 4, 5
 ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE = 0, 1, 2
 
 # Every exported symbol and its ctypes signature (restype, argtypes).
@@ -44,6 +44,7 @@ SIGNATURES = {
 }
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
+ED_Q_MIXED_F32_FLOPS_PER_T, ED_Q_MIXED_F16_FLOPS_PER_T = 7, 8
 
 _lib: Optional[ctypes.CDLL] = None
 

@@ -133,6 +133,9 @@ struct ed_context {
   DevBuf<double> psum;  // [npb*16] sum over sites of Phi_b
   DevBuf<double> ph64;
   DevBuf<float> ph32;
+  DevBuf<ed::half8> ph16;  // [npb][V3P/32][4][16][32] fp16 hi/lo of 2^14 Phi
+  int mixed_stage2 = -1;   // 1: FP16 three-term split, 0: FP32 MFMA, -1: FP16 from V3 >= 4096 on
+  int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
 
   // workspace
   int tbatch_opt = 0;
@@ -341,6 +344,23 @@ void build_plan(ed_context* c) {
         }
   }
   std::vector<float> pr32(pr.begin(), pr.end()), pi32(pi.begin(), pi.end());
+  // FP16 hi/lo split of 2^14 Phi for the mixed form's FP16 MFMA phase contraction
+  const int nchunk2 = c->V3P / ed::SC2;
+  std::vector<_Float16> p16((size_t)c->npb * nchunk2 * 4 * 16 * 32);
+  for (int blk = 0; blk < c->npb; ++blk)
+    for (int ch = 0; ch < nchunk2; ++ch)
+      for (int col = 0; col < 16; ++col)
+        for (int xs = 0; xs < 32; ++xs) {
+          const size_t src = ((size_t)blk * c->V3P + (size_t)ch * 32 + xs) * 16 + col;
+          const double vr = pr[src] * 16384.0, vi = pi[src] * 16384.0;
+          const _Float16 rh = (_Float16)vr, ih = (_Float16)vi;
+          const _Float16 rl = (_Float16)(vr - (double)rh), il = (_Float16)(vi - (double)ih);
+          const size_t base = (((size_t)blk * nchunk2 + ch) * 4) * 16 * 32 + (size_t)col * 32 + xs;
+          p16[base + 0 * 512] = rh;
+          p16[base + 1 * 512] = rl;
+          p16[base + 2 * 512] = ih;
+          p16[base + 3 * 512] = il;
+        }
   // sum_x Phi_b(x) is V3 when every component of b is a multiple of L, else 0
   std::vector<double> ps(NB, 0.0);
   for (int i = 0; i < c->nbase; ++i)
@@ -352,6 +372,7 @@ void build_plan(ed_context* c) {
   c->psum.ensure(ps.size());
   c->ph64.ensure(2 * pr.size());
   c->ph32.ensure(2 * pr.size());
+  c->ph16.ensure(p16.size() / 8);
   HIPCHK(hipMemcpy(c->dplan.p, dpl.data(), dpl.size() * sizeof(DPlan), hipMemcpyHostToDevice));
   if (!sp.empty()) HIPCHK(hipMemcpy(c->splits.p, sp.data(), sp.size() * sizeof(SplitTerm), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->pidx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -360,6 +381,7 @@ void build_plan(ed_context* c) {
   HIPCHK(hipMemcpy(c->ph64.p + pr.size(), pi.data(), pi.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->ph32.p, pr32.data(), pr32.size() * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->ph32.p + pr32.size(), pi32.data(), pi32.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ph16.p, p16.data(), p16.size() * sizeof(_Float16), hipMemcpyHostToDevice));
   c->plan_dirty = false;
 }
 
@@ -393,10 +415,12 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
   }
   TB = std::min(TB, nt);
   const size_t ws = per_t * TB;
-  if (c->W.n < ws) {
+  if (c->W.n < ws || c->W_elem != (int)sizeof(ed::c_t<T>)) {
     c->W.ensure(ws);
-    // padded rows and sites must hold zeros; they are never written later
-    HIPCHK(hipMemsetAsync(c->W.p, 0, ws, c->stream));
+    // padded rows and sites must hold zeros (they are never written); a
+    // buffer last used with the other element type is cleared again
+    HIPCHK(hipMemsetAsync(c->W.p, 0, c->W.n, c->stream));
+    c->W_elem = (int)sizeof(ed::c_t<T>);
   }
   ed::c_t<T>* W = (ed::c_t<T>*)(void*)c->W.p;
   const Geom g = c->geom(c->D);
@@ -444,6 +468,22 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     a.wgt = c->use_wgt ? c->wgt.p : nullptr;
     a.psum = c->psum.p;
     const int nblk = (a.nwork + 7) / 8 * 8;
+    const bool f16_stage2 = c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096);
+    if (sizeof(T) == 4 && f16_stage2) {
+      const float2* Wf = (const float2*)(const void*)W;
+      c->launch(ED_KERNEL_CONTRACT, [&] {
+        if (c->nks_max <= 1)
+          ed::contract_mixed_f16_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+        else if (c->nks_max <= 2)
+          ed::contract_mixed_f16_kernel<2><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+        else if (c->nks_max <= 3)
+          ed::contract_mixed_f16_kernel<3><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+        else if (c->nks_max <= 6)
+          ed::contract_mixed_f16_kernel<6><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+        else
+          fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", 4 * c->nks_max / 3);
+      });
+    } else
     c->launch(ED_KERNEL_CONTRACT, [&] {
       if (c->nks_max <= 1)
         ed::contract_kernel<T, 1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(W, ph, ph + phn, out, a);
@@ -543,13 +583,14 @@ int ed_set_lattice(ed_context* ctx, int Lx, int Ly, int Lz, int Lt, int Ne) {
     ctx->Lt = Lt;
     ctx->Ne = Ne;
     ctx->V3 = (int)v3;
-    ctx->V3P = (int)((v3 + ed::SC - 1) / ed::SC * ed::SC);
+    ctx->V3P = (int)((v3 + ed::SC2 - 1) / ed::SC2 * ed::SC2);  // multiple of both chunk sizes
     ctx->NeP = (Ne + 15) / 16 * 16;
     ctx->lattice_set = true;
     ctx->gauge_set = ctx->eig_set = false;
     ctx->use_wgt = false;
     ctx->plan_dirty = true;
     ctx->W.reset();
+    ctx->W_elem = 0;
     ctx->stage.reset();
   });
 }
@@ -629,6 +670,7 @@ int ed_set_derivatives(ed_context* ctx, int num_nabla) {
     ctx->D = level_start(num_nabla + 1);
     ctx->plan_dirty = true;
     ctx->W.reset();
+    ctx->W_elem = 0;
     ctx->stage.reset();
   });
 }
@@ -668,6 +710,10 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
       case ED_OPT_PROFILE:
         ctx->profile = value ? 1 : 0;
         break;
+      case ED_OPT_MIXED_STAGE2:
+        if (value < -1 || value > 1) fail(ED_ERR_ARG, "mixed stage-2 kind must be -1 (auto), 0 (FP32) or 1 (FP16 split)");
+        ctx->mixed_stage2 = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown option %d", option);
     }
@@ -705,6 +751,17 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
         // stage 2 (4 waves x 4 k-steps x 4 N-blocks x 4 real products)
         const int64_t mf = ctx->npb * tiles * chunks * (ctx->mfma_per_chunk_stage1 + (int64_t)ctx->nd * 256);
         *value = mf * 2 * 16 * 16 * 4;
+        break;
+      }
+      case ED_Q_MIXED_F32_FLOPS_PER_T:
+      case ED_Q_MIXED_F16_FLOPS_PER_T: {
+        // the mixed FP16 kernel: per (tile, d, 32-site chunk) stage 1 has 32 * 4 * nks(d)
+        // FP32 MFMAs 16x16x4 and stage 2 has 16 N-blocks x 12 FP16 MFMAs 16x16x32
+        const int64_t chunks2 = ctx->V3P / ed::SC2;
+        if (what == ED_Q_MIXED_F32_FLOPS_PER_T)
+          *value = ctx->npb * tiles * chunks2 * (2 * ctx->mfma_per_chunk_stage1) * 2 * 16 * 16 * 4;
+        else
+          *value = ctx->npb * tiles * chunks2 * (int64_t)ctx->nd * 16 * 12 * 2 * 16 * 16 * 32;
         break;
       }
       case ED_Q_ALGO_FLOPS_PER_T:

@@ -477,4 +477,279 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* _
   }
 }
 
+// --------------------------------------------------------------------------
+// Mixed form with the phase contraction on FP16 MFMA.
+//
+// Stage 1 is the FP32 MFMA of contract_kernel<float> (4 sites per wave,
+// 32-site chunks).  Stage 2 runs v_mfma_f32_16x16x32_f16 (K = 32 sites) on a
+// three-term split: with power-of-two scales sq (Q) and 2^14 (Phi), each
+// operand x is held as hi = fp16(x s), lo = fp16(x s - hi), and
+//   Phi Q ~ (Phi_hi Q_hi + Phi_hi Q_lo + Phi_lo Q_hi) / (2^14 sq),
+// i.e. a 22-bit representation of both operands with FP32 accumulation over
+// the 32 sites of a chunk, then FP64 across chunks.  sq is chosen per chunk
+// from the tile maximum of |Q| (gathered through LDS before the barrier), so
+// no value can overflow FP16 and all values within 2^-17 of the maximum keep
+// their full 22 bits.
+// --------------------------------------------------------------------------
+constexpr int SC2 = 32;            // sites per chunk (one K = 32 MFMA)
+constexpr int SPW2 = SC2 / NWAVE;  // stage-1 sites per wave and chunk
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
+}
+
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const float2* __restrict__ W,
+                                                                            const half8* __restrict__ ph16,
+                                                                            double2* __restrict__ out,
+                                                                            ContractArgs a) {
+  typedef float4v acc_t;
+  __shared__ float2 Qs[2][SC2][16][16];
+  __shared__ float wmax[2][NWAVE];
+
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= a.nwork) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g4 = lane >> 4;
+
+  int rest = w;
+  const int tf = rest % a.ntf; rest /= a.ntf;
+  const int te = rest % a.nte; rest /= a.nte;
+  const int pb = rest % a.npb; rest /= a.npb;
+  const int di = rest % a.nd;
+  const int tb = rest / a.nd;
+  const DPlan dp = a.dplan[di];
+  const int E0 = te * 16, F0 = tf * 16;
+  const Geom g = a.g;
+  const size_t site = (size_t)3 * g.NeP;
+  const float2* Wt = W + (size_t)tb * g.nW * g.V3P * site;
+  const int J = 3 * dp.nsplit;
+  const int nks = (J + 3) >> 2;
+  const int nchunk = g.V3P / SC2;
+
+  const float2* Lb[NKS];
+  const float2* Rb[NKS];
+  float cf[NKS];
+#pragma unroll
+  for (int ks = 0; ks < NKS; ++ks) {
+    const int j = 4 * ks + g4;
+    if (ks < nks && j < J) {
+      const SplitTerm st = a.splits[dp.first + j / 3];
+      const int c = j % 3;
+      Lb[ks] = Wt + widx(g, st.nL, 0, c, E0 + l16);
+      Rb[ks] = Wt + widx(g, st.nR, 0, c, F0 + l16);
+      cf[ks] = st.coeff;
+    } else {
+      Lb[ks] = Wt;
+      Rb[ks] = Wt;
+      cf[ks] = 0.f;
+    }
+  }
+  // phase operands: [npb][nchunk][4 parts][16 momenta][32 sites], parts =
+  // (Re hi, Re lo, Im hi, Im lo); lane (g4, l16) reads momentum l16, sites 8 g4..8 g4+7
+  const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4 + (size_t)l16 * 4 + g4;
+
+  float2 lv[NKS][SPW2], rv[NKS][SPW2];
+  half8 prh, prl, pih, pil;
+  auto load_ops = [&](int c0) {
+    const int xb = c0 + SPW2 * wave;
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      if (ks < nks) {
+#pragma unroll
+        for (int i = 0; i < SPW2; ++i) {
+          lv[ks][i] = Lb[ks][(size_t)(xb + i) * site];
+          rv[ks][i] = Rb[ks][(size_t)(xb + i) * site];
+        }
+      }
+    }
+  };
+  auto load_phase = [&](int chunk) {
+    const half8* p = PH + (size_t)chunk * 4 * 16 * 4;
+    prh = p[0 * 64];
+    prl = p[1 * 64];
+    pih = p[2 * 64];
+    pil = p[3 * 64];
+  };
+
+  acc_t A1[NBW], A2[NBW], A3[NBW], A4[NBW];
+  double D1[NBW][4], D2[NBW][4], D3[NBW][4], D4[NBW][4];
+  float zr[NBW], zi[NBW];
+  double Zr[NBW], Zi[NBW];
+  float cr[NBW], ci[NBW];
+#pragma unroll
+  for (int b = 0; b < NBW; ++b) {
+    A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
+    zr[b] = zi[b] = 0;
+    Zr[b] = Zi[b] = 0;
+    cr[b] = ci[b] = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
+  }
+
+  load_ops(0);
+  load_phase(0);
+  int buf = 0;
+  for (int chunk = 0; chunk < nchunk; ++chunk, buf ^= 1) {
+    const int c0 = chunk * SC2;
+    // ---- stage 1 (FP32 MFMA) ----
+    acc_t qr[SPW2], qi[SPW2];
+#pragma unroll
+    for (int i = 0; i < SPW2; ++i) qr[i] = qi[i] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+      if (ks < nks) {
+#pragma unroll
+        for (int i = 0; i < SPW2; ++i) {
+          const float lr = cf[ks] * lv[ks][i].x, li = cf[ks] * lv[ks][i].y;
+          qr[i] = mfma<float>::mac(lr, rv[ks][i].x, qr[i]);
+          qr[i] = mfma<float>::mac(li, rv[ks][i].y, qr[i]);
+          qi[i] = mfma<float>::mac(lr, rv[ks][i].y, qi[i]);
+          qi[i] = mfma<float>::mac(-li, rv[ks][i].x, qi[i]);
+        }
+      }
+    }
+    if (chunk + 1 < nchunk) load_ops(c0 + SC2);
+    float m = 0.f;
+#pragma unroll
+    for (int i = 0; i < SPW2; ++i) {
+      const bool real_site = (c0 + SPW2 * wave + i) < g.V3;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Qs[buf][SPW2 * wave + i][mfma<float>::row(g4, r)][l16] = make_float2(qr[i][r], qi[i][r]);
+        if (real_site) m = fmaxf(m, fmaxf(fabsf(qr[i][r]), fabsf(qi[i][r])));
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (lane == 0) wmax[buf][wave] = m;
+    __syncthreads();
+
+    if (chunk == 0) {
+      const int n0 = min(SC2, g.V3);
+#pragma unroll
+      for (int b = 0; b < NBW; ++b) {
+        float sr = 0, si = 0;
+        for (int xl = 0; xl < n0; ++xl) {
+          const float2 q = Qs[buf][xl][NBW * wave + b][l16];
+          sr += q.x;
+          si += q.y;
+        }
+        cr[b] = sr / (float)n0;
+        ci[b] = si / (float)n0;
+      }
+    }
+    // chunk scale: |Q - c| <= 2 max|Q| < 2^ex, scaled values stay below 2^14
+    float tmax = 0.f;
+#pragma unroll
+    for (int v = 0; v < NWAVE; ++v) tmax = fmaxf(tmax, wmax[buf][v]);
+    int ex = 0;
+    if (tmax > 0.f) (void)frexpf(2.f * tmax, &ex);
+    const float sq = ldexpf(1.f, 14 - ex);
+
+    // ---- stage 2 (FP16 MFMA, three-term split) ----
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) {
+      half8 qrh, qrl, qih, qil;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int xl = 8 * g4 + v;
+        float2 q = Qs[buf][xl][NBW * wave + b][l16];
+        const bool real_site = (c0 + xl) < g.V3;
+        q.x = real_site ? q.x - cr[b] : 0.f;
+        q.y = real_site ? q.y - ci[b] : 0.f;
+        zr[b] += q.x;
+        zi[b] += q.y;
+        _Float16 h, l;
+        split16(q.x * sq, h, l);
+        qrh[v] = h;
+        qrl[v] = l;
+        split16(q.y * sq, h, l);
+        qih[v] = h;
+        qil[v] = l;
+      }
+      A1[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qrh, A1[b], 0, 0, 0);
+      A1[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qrl, A1[b], 0, 0, 0);
+      A1[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, qrh, A1[b], 0, 0, 0);
+      A2[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qih, A2[b], 0, 0, 0);
+      A2[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qil, A2[b], 0, 0, 0);
+      A2[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, qih, A2[b], 0, 0, 0);
+      A3[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qrh, A3[b], 0, 0, 0);
+      A3[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qrl, A3[b], 0, 0, 0);
+      A3[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, qrh, A3[b], 0, 0, 0);
+      A4[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qih, A4[b], 0, 0, 0);
+      A4[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qil, A4[b], 0, 0, 0);
+      A4[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, qih, A4[b], 0, 0, 0);
+    }
+    if (chunk + 1 < nchunk) load_phase(chunk + 1);
+    // FP32 chunk sums into FP64, unscaled exactly (powers of two)
+    const double inv = ldexp(1.0, ex - 28);  // 1 / (2^14 sq)
+#pragma unroll
+    for (int b = 0; b < NBW; ++b) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        D1[b][r] += (double)A1[b][r] * inv;
+        D2[b][r] += (double)A2[b][r] * inv;
+        D3[b][r] += (double)A3[b][r] * inv;
+        D4[b][r] += (double)A4[b][r] * inv;
+      }
+      A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
+      Zr[b] += (double)zr[b];
+      Zi[b] += (double)zi[b];
+      zr[b] = zi[b] = 0;
+    }
+  }
+
+  // ---- epilogue (as contract_kernel) ----
+  const int Ne = g.Ne, P = a.P;
+  const size_t plane = (size_t)Ne * Ne;
+  const size_t tbase = (size_t)(a.t_out0 + tb) * a.D * P;
+  const int f = F0 + l16;
+#pragma unroll
+  for (int b = 0; b < NBW; ++b) {
+    const int e = E0 + NBW * wave + b;
+    const bool ok = (e < Ne) && (f < Ne);
+    const double wef = (ok && a.wgt) ? a.wgt[(size_t)e * Ne + f] : 1.0;
+    const double wfe = (ok && a.wgt) ? a.wgt[(size_t)f * Ne + e] : 1.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int bi = pb * 16 + mfma<float>::row(g4, r);
+      if (!ok || bi >= a.nbase) continue;
+      const int ip = a.plus_idx[bi], im = a.minus_idx[bi];
+      const double S = a.psum[bi], mr = (double)cr[b] * S, mi = (double)ci[b] * S;
+      const double epr = D1[b][r] - D2[b][r] + mr, epi = D3[b][r] + D4[b][r] + mi;
+      const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
+      if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
+      if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        if (im >= 0)
+          out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
+              make_double2(s * epr * wfe, -s * epi * wfe);
+        if (ip >= 0)
+          out[(tbase + (size_t)dp.partner * P + ip) * plane + (size_t)f * Ne + e] =
+              make_double2(s * emr * wfe, -s * emi * wfe);
+      }
+    }
+    double zre = Zr[b], zim = Zi[b];
+    zre += __shfl_xor(zre, 16);
+    zim += __shfl_xor(zim, 16);
+    zre += __shfl_xor(zre, 32);
+    zim += __shfl_xor(zim, 32);
+    zre += (double)cr[b] * g.V3;
+    zim += (double)ci[b] * g.V3;
+    if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
+      out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
+            make_double2(s * zre * wfe, -s * zim * wfe);
+      }
+    }
+  }
+}
+
 }  // namespace ed

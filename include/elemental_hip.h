@@ -110,8 +110,18 @@ int ed_synchronize(ed_context* ctx);
  *                         E_{rev d}[p] = (-1)^{|d|} E_d[-p]^dagger; 0: compute all d.
  *   ED_OPT_TBATCH         timeslices per batch (0 = automatic).
  *   ED_OPT_WS_MB          workspace budget in MiB for the automatic batch.
- *   ED_OPT_PROFILE        1: record HIP events around every kernel launch. */
-enum ed_option { ED_OPT_HERMITIAN = 1, ED_OPT_TBATCH = 2, ED_OPT_WS_MB = 3, ED_OPT_PROFILE = 4 };
+ *   ED_OPT_PROFILE        1: record HIP events around every kernel launch.
+ *   ED_OPT_MIXED_STAGE2   mixed form's phase contraction: 1 FP16 MFMA on a three-term
+ *                         hi/lo split, 0 FP32 MFMA, -1 (default) FP16 when V3 >= 4096
+ *                         (small lattices keep FP32, whose error on blocks that are small
+ *                         against their terms is ~2.5x lower). */
+enum ed_option {
+  ED_OPT_HERMITIAN = 1,
+  ED_OPT_TBATCH = 2,
+  ED_OPT_WS_MB = 3,
+  ED_OPT_PROFILE = 4,
+  ED_OPT_MIXED_STAGE2 = 5
+};
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 
 /* Kernel timing collected while ED_OPT_PROFILE = 1, per kernel family:
@@ -133,14 +143,19 @@ int ed_profile_reset(ed_context* ctx);
  *                              Hermiticity)
  *   ED_Q_MOMENTUM_BLOCKS       blocks of 16 base momenta
  *   ED_Q_HOPS_PER_T            covariant hops per timeslice (D - 1)
- *   ED_Q_SPLITS                split terms over all derivatives (S) */
+ *   ED_Q_SPLITS                split terms over all derivatives (S)
+ *   ED_Q_MIXED_F32_FLOPS_PER_T FP32 MFMA flops of the mixed FP16-stage-2 kernel per timeslice
+ *   ED_Q_MIXED_F16_FLOPS_PER_T FP16 MFMA flops of the same kernel per timeslice
+ * (ED_Q_CONTRACT_FLOPS_PER_T counts the FP64 / FP32-stage-2 kernels.) */
 enum ed_query {
   ED_Q_CONTRACT_FLOPS_PER_T = 1,
   ED_Q_ALGO_FLOPS_PER_T = 2,
   ED_Q_PRIMARY_DERIVATIVES = 3,
   ED_Q_MOMENTUM_BLOCKS = 4,
   ED_Q_HOPS_PER_T = 5,
-  ED_Q_SPLITS = 6
+  ED_Q_SPLITS = 6,
+  ED_Q_MIXED_F32_FLOPS_PER_T = 7,
+  ED_Q_MIXED_F16_FLOPS_PER_T = 8
 };
 int ed_query(ed_context* ctx, int what, int64_t* value);
 

@@ -218,3 +218,19 @@ def test_config_b_shape_properties():
     Eref0 = O.elemental_timeslice(Vh[0], Uh[:, 0], latt, 2, sub)
     got = E[:1][:, :, [idx[m] for m in sub]]
     assert blockwise(got, Eref0[None]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("stage2", [0, 1])
+def test_mixed_stage2_variants(stage2):
+    """Both phase-contraction variants of the mixed form (FP32 MFMA, FP16 split) at 16^3."""
+    latt, Ne = [16, 16, 16, 1], 24
+    moms = S.momenta_by_norm(4)
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=77)
+    say(f"mixed stage-2 variant {stage2}")
+    g = ed.ElementalGenerator(latt, None, None, 2, moms, Ne=Ne, mode="mixed")
+    g.set_option(_native.ED_OPT_MIXED_STAGE2, stage2)
+    g.load_arrays(U, V)
+    E = g.calc_all()
+    err = blockwise(E, O.elemental(V, U, latt, 2, moms))
+    say(f"  err={err:.3e}")
+    assert err <= MIXED_TOL
