@@ -468,7 +468,9 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     a.wgt = c->use_wgt ? c->wgt.p : nullptr;
     a.psum = c->psum.p;
     const int nblk = (a.nwork + 7) / 8 * 8;
-    const bool f16_stage2 = c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096);
+    // the FP16 kernel addresses one timeslice's workspace with 32-bit buffer offsets
+    const bool fits32 = per_t_ws_bytes<float>(c) < ((size_t)1 << 31);
+    const bool f16_stage2 = fits32 && (c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096));
     if (sizeof(T) == 4 && f16_stage2) {
       const float2* Wf = (const float2*)(const void*)W;
       c->launch(ED_KERNEL_CONTRACT, [&] {

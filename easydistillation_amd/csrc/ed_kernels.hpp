@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ed {
 
 template <typename T> struct cplx;
@@ -501,18 +503,50 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - (float)hi);
 }
 
+// wave maximum of a non-negative value through DPP row shifts and row
+// broadcasts (no LDS traffic); every lane receives the result
+template <int CTRL, int RMASK>
+__device__ __forceinline__ float dpp_max_step(float x) {
+  const int y = __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, RMASK, 0xF, false);
+  return fmaxf(x, __int_as_float(y));
+}
+__device__ __forceinline__ float wave_max(float v) {
+  v = dpp_max_step<0x111, 0xF>(v);  // row_shr:1
+  v = dpp_max_step<0x112, 0xF>(v);  // row_shr:2
+  v = dpp_max_step<0x114, 0xF>(v);  // row_shr:4
+  v = dpp_max_step<0x118, 0xF>(v);  // row_shr:8
+  v = dpp_max_step<0x142, 0xA>(v);  // row_bcast:15
+  v = dpp_max_step<0x143, 0xC>(v);  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Three-term split of two FP32 values: hi = fp16(x) (round to nearest even,
+// packed), lo = fp16(x - hi) with the residual formed exactly inside
+// v_fma_mix from the FP16 hi and the FP32 x.
+__device__ __forceinline__ void split16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  const h2 h = {(_Float16)x0, (_Float16)x1};
+  hi = __builtin_bit_cast(uint32_t, h);
+  uint32_t l;
+  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(x0));
+  asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hi), "v"(x1));
+  lo = l;
+}
+
 template <int NKS>
 __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const float2* __restrict__ W,
                                                                             const half8* __restrict__ ph16,
                                                                             double2* __restrict__ out,
                                                                             ContractArgs a) {
   typedef float4v acc_t;
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
   __shared__ float2 Qs[2][SC2][16][16];
   __shared__ float wmax[2][NWAVE];
 
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;
 
   int rest = w;
@@ -524,14 +558,16 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
   const DPlan dp = a.dplan[di];
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
-  const size_t site = (size_t)3 * g.NeP;
-  const float2* Wt = W + (size_t)tb * g.nW * g.V3P * site;
-  const int J = 3 * dp.nsplit;
-  const int nks = (J + 3) >> 2;
+  const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;  // bytes per site
+  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^32 (checked on the host)
+  const float2* Wt = W + (size_t)tb * (slice_b / 8);
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
+  const int J = __builtin_amdgcn_readfirstlane(3 * dp.nsplit);
+  const int nks = (J + 3) >> 2;  // uniform
   const int nchunk = g.V3P / SC2;
 
-  const float2* Lb[NKS];
-  const float2* Rb[NKS];
+  // per k-step byte offsets of site 0 (within the timeslice block) and coefficients
+  int offL[NKS], offR[NKS];
   float cf[NKS];
 #pragma unroll
   for (int ks = 0; ks < NKS; ++ks) {
@@ -539,109 +575,189 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
     if (ks < nks && j < J) {
       const SplitTerm st = a.splits[dp.first + j / 3];
       const int c = j % 3;
-      Lb[ks] = Wt + widx(g, st.nL, 0, c, E0 + l16);
-      Rb[ks] = Wt + widx(g, st.nR, 0, c, F0 + l16);
+      offL[ks] = (int)(widx(g, st.nL, 0, c, E0 + l16) * 8);
+      offR[ks] = (int)(widx(g, st.nR, 0, c, F0 + l16) * 8);
       cf[ks] = st.coeff;
     } else {
-      Lb[ks] = Wt;
-      Rb[ks] = Wt;
+      offL[ks] = offR[ks] = 0;
       cf[ks] = 0.f;
     }
   }
   // phase operands: [npb][nchunk][4 parts][16 momenta][32 sites], parts =
   // (Re hi, Re lo, Im hi, Im lo); lane (g4, l16) reads momentum l16, sites 8 g4..8 g4+7
-  const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4 + (size_t)l16 * 4 + g4;
+  const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4;
+  const __amdgpu_buffer_rsrc_t pr_ = __builtin_amdgcn_make_buffer_rsrc((void*)PH, (short)0, nchunk * 4 * 16 * 4 * 16, 0x00020000);
+  const int phoff = (l16 * 4 + g4) * 16;
 
-  float2 lv[NKS][SPW2], rv[NKS][SPW2];
+  v2f lv[NKS][SPW2], rv[NKS][SPW2];
   half8 prh, prl, pih, pil;
   auto load_ops = [&](int c0) {
-    const int xb = c0 + SPW2 * wave;
+    const int xo = (c0 + SPW2 * wave) * (int)site_b;  // uniform
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
       if (ks < nks) {
 #pragma unroll
         for (int i = 0; i < SPW2; ++i) {
-          lv[ks][i] = Lb[ks][(size_t)(xb + i) * site];
-          rv[ks][i] = Rb[ks][(size_t)(xb + i) * site];
+          lv[ks][i] = __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(wr, offL[ks], xo + i * (int)site_b, 0));
+          rv[ks][i] = __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(wr, offR[ks], xo + i * (int)site_b, 0));
         }
       }
     }
   };
   auto load_phase = [&](int chunk) {
-    const half8* p = PH + (size_t)chunk * 4 * 16 * 4;
-    prh = p[0 * 64];
-    prl = p[1 * 64];
-    pih = p[2 * 64];
-    pil = p[3 * 64];
+    const int co = chunk * 4 * 16 * 4 * 16;  // uniform
+    prh = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 0 * 1024, 0));
+    prl = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 1 * 1024, 0));
+    pih = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 2 * 1024, 0));
+    pil = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 3 * 1024, 0));
   };
 
-  acc_t A1[NBW], A2[NBW], A3[NBW], A4[NBW];
+  const acc_t zero = {0.f, 0.f, 0.f, 0.f};
   double D1[NBW][4], D2[NBW][4], D3[NBW][4], D4[NBW][4];
-  float zr[NBW], zi[NBW];
   double Zr[NBW], Zi[NBW];
   float cr[NBW], ci[NBW];
 #pragma unroll
   for (int b = 0; b < NBW; ++b) {
-    A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
-    zr[b] = zi[b] = 0;
     Zr[b] = Zi[b] = 0;
     cr[b] = ci[b] = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
   }
 
-  load_ops(0);
-  load_phase(0);
-  int buf = 0;
-  for (int chunk = 0; chunk < nchunk; ++chunk, buf ^= 1) {
+  // stage 2 of one chunk for N-block b; FULL: no padded site in the chunk
+  auto stage2 = [&](auto FULL, int b, int c0, float sq, int ex) {
+    constexpr bool full = decltype(FULL)::value;
+    const float csr = -cr[b] * sq, csi = -ci[b] * sq;
+    uint32_t qrh[4], qrl[4], qih[4], qil[4];
+    float zsr = 0.f, zsi = 0.f;
+#pragma unroll
+    for (int v2 = 0; v2 < 4; ++v2) {
+      float sr[2], si[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int xl = 8 * g4 + 2 * v2 + h;
+        const float2 q = Qs[(c0 / SC2) & 1][xl][NBW * wave + b][l16];
+        // scaled and mean-subtracted value s = (q - c) sq; padded sites give 0
+        sr[h] = fmaf(q.x, sq, csr);
+        si[h] = fmaf(q.y, sq, csi);
+        if constexpr (!full) {
+          if ((c0 + xl) >= g.V3) sr[h] = si[h] = 0.f;
+        }
+        zsr += sr[h];
+        zsi += si[h];
+      }
+      split16x2(sr[0], sr[1], qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], qih[v2], qil[v2]);
+    }
+    const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
+    const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
+    const half8 Qih = __builtin_bit_cast(half8, u4{qih[0], qih[1], qih[2], qih[3]});
+    const half8 Qil = __builtin_bit_cast(half8, u4{qil[0], qil[1], qil[2], qil[3]});
+    acc_t A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qrh, zero, 0, 0, 0);
+    acc_t A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qih, zero, 0, 0, 0);
+    acc_t A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qrh, zero, 0, 0, 0);
+    acc_t A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qih, zero, 0, 0, 0);
+    A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qrl, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qil, A2, 0, 0, 0);
+    A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qrl, A3, 0, 0, 0);
+    A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qil, A4, 0, 0, 0);
+    A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, Qrh, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, Qih, A2, 0, 0, 0);
+    A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, Qrh, A3, 0, 0, 0);
+    A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, Qih, A4, 0, 0, 0);
+    // FP32 chunk sums into FP64, unscaled exactly (powers of two)
+    const double inv = ldexp(1.0, ex - 28);  // 1 / (2^14 sq)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      D1[b][r] = fma((double)A1[r], inv, D1[b][r]);
+      D2[b][r] = fma((double)A2[r], inv, D2[b][r]);
+      D3[b][r] = fma((double)A3[r], inv, D3[b][r]);
+      D4[b][r] = fma((double)A4[r], inv, D4[b][r]);
+    }
+    const double invq = ldexp(1.0, ex - 14);  // 1 / sq
+    Zr[b] = fma((double)zsr, invq, Zr[b]);
+    Zi[b] = fma((double)zsi, invq, Zi[b]);
+  };
+
+  // stage 1 of chunk `chunk` from the prefetched operands into Qs[buf] and
+  // wmax[buf]; the operands of chunk + 1 are requested right after use
+  auto stage1 = [&](int chunk, int buf) {
     const int c0 = chunk * SC2;
-    // ---- stage 1 (FP32 MFMA) ----
     acc_t qr[SPW2], qi[SPW2];
+    float lr[SPW2], li[SPW2];
 #pragma unroll
-    for (int i = 0; i < SPW2; ++i) qr[i] = qi[i] = acc_t{0, 0, 0, 0};
+    for (int i = 0; i < SPW2; ++i) {
+      lr[i] = cf[0] * lv[0][i].x;
+      li[i] = cf[0] * lv[0][i].y;
+    }
+    // conj(L) R = (lr rr + li ri) + i (lr ri - li rr); dependent MFMAs are
+    // separated by independent ones
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
+    for (int i = 0; i < SPW2; ++i) qr[i] = mfma<float>::mac(lr[i], rv[0][i].x, zero);
+#pragma unroll
+    for (int i = 0; i < SPW2; ++i) qi[i] = mfma<float>::mac(lr[i], rv[0][i].y, zero);
+#pragma unroll
+    for (int i = 0; i < SPW2; ++i) qr[i] = mfma<float>::mac(li[i], rv[0][i].y, qr[i]);
+#pragma unroll
+    for (int i = 0; i < SPW2; ++i) qi[i] = mfma<float>::mac(-li[i], rv[0][i].x, qi[i]);
+#pragma unroll
+    for (int ks = 1; ks < NKS; ++ks) {
       if (ks < nks) {
 #pragma unroll
         for (int i = 0; i < SPW2; ++i) {
-          const float lr = cf[ks] * lv[ks][i].x, li = cf[ks] * lv[ks][i].y;
-          qr[i] = mfma<float>::mac(lr, rv[ks][i].x, qr[i]);
-          qr[i] = mfma<float>::mac(li, rv[ks][i].y, qr[i]);
-          qi[i] = mfma<float>::mac(lr, rv[ks][i].y, qi[i]);
-          qi[i] = mfma<float>::mac(-li, rv[ks][i].x, qi[i]);
+          lr[i] = cf[ks] * lv[ks][i].x;
+          li[i] = cf[ks] * lv[ks][i].y;
         }
+#pragma unroll
+        for (int i = 0; i < SPW2; ++i) qr[i] = mfma<float>::mac(lr[i], rv[ks][i].x, qr[i]);
+#pragma unroll
+        for (int i = 0; i < SPW2; ++i) qi[i] = mfma<float>::mac(lr[i], rv[ks][i].y, qi[i]);
+#pragma unroll
+        for (int i = 0; i < SPW2; ++i) qr[i] = mfma<float>::mac(li[i], rv[ks][i].y, qr[i]);
+#pragma unroll
+        for (int i = 0; i < SPW2; ++i) qi[i] = mfma<float>::mac(-li[i], rv[ks][i].x, qi[i]);
       }
     }
     if (chunk + 1 < nchunk) load_ops(c0 + SC2);
+    const bool full = c0 + SC2 <= g.V3;  // uniform: no padded site in this chunk
     float m = 0.f;
 #pragma unroll
     for (int i = 0; i < SPW2; ++i) {
-      const bool real_site = (c0 + SPW2 * wave + i) < g.V3;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        Qs[buf][SPW2 * wave + i][mfma<float>::row(g4, r)][l16] = make_float2(qr[i][r], qi[i][r]);
-        if (real_site) m = fmaxf(m, fmaxf(fabsf(qr[i][r]), fabsf(qi[i][r])));
+      for (int r = 0; r < 4; ++r) Qs[buf][SPW2 * wave + i][mfma<float>::row(g4, r)][l16] = make_float2(qr[i][r], qi[i][r]);
+      if (full || (c0 + SPW2 * wave + i) < g.V3) {
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(qr[i][0]), fabsf(qi[i][0])), fmaxf(fabsf(qr[i][1]), fabsf(qi[i][1]))));
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(qr[i][2]), fabsf(qi[i][2])), fmaxf(fabsf(qr[i][3]), fabsf(qi[i][3]))));
       }
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    m = wave_max(m);
     if (lane == 0) wmax[buf][wave] = m;
-    __syncthreads();
+  };
 
-    if (chunk == 0) {
-      const int n0 = min(SC2, g.V3);
+  // Software pipeline: the barrier interval of chunk k holds stage 2 of
+  // chunk k (reading Qs[k & 1]) and stage 1 of chunk k + 1 (writing
+  // Qs[(k + 1) & 1]), so the MFMA pipe has independent work while LDS reads
+  // and conversions of the other stage are in flight.
+  load_ops(0);
+  load_phase(0);
+  stage1(0, 0);
+  __syncthreads();
+  {
+    const int n0 = min(SC2, g.V3);
 #pragma unroll
-      for (int b = 0; b < NBW; ++b) {
-        float sr = 0, si = 0;
-        for (int xl = 0; xl < n0; ++xl) {
-          const float2 q = Qs[buf][xl][NBW * wave + b][l16];
-          sr += q.x;
-          si += q.y;
-        }
-        cr[b] = sr / (float)n0;
-        ci[b] = si / (float)n0;
+    for (int b = 0; b < NBW; ++b) {
+      float sr = 0, si = 0;
+      for (int xl = 0; xl < n0; ++xl) {
+        const float2 q = Qs[0][xl][NBW * wave + b][l16];
+        sr += q.x;
+        si += q.y;
       }
+      cr[b] = sr / (float)n0;
+      ci[b] = si / (float)n0;
     }
+  }
+  for (int chunk = 0; chunk < nchunk; ++chunk) {
+    const int c0 = chunk * SC2, buf = chunk & 1;
     // chunk scale: |Q - c| <= 2 max|Q| < 2^ex, scaled values stay below 2^14
     float tmax = 0.f;
 #pragma unroll
@@ -650,57 +766,18 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
     if (tmax > 0.f) (void)frexpf(2.f * tmax, &ex);
     const float sq = ldexpf(1.f, 14 - ex);
 
-    // ---- stage 2 (FP16 MFMA, three-term split) ----
+    if (chunk + 1 < nchunk) stage1(chunk + 1, buf ^ 1);
+
+    // ---- stage 2 (FP16 MFMA, three-term split) of chunk ----
+    if (c0 + SC2 <= g.V3) {
 #pragma unroll
-    for (int b = 0; b < NBW; ++b) {
-      half8 qrh, qrl, qih, qil;
+      for (int b = 0; b < NBW; ++b) stage2(std::true_type{}, b, c0, sq, ex);
+    } else {
 #pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int xl = 8 * g4 + v;
-        float2 q = Qs[buf][xl][NBW * wave + b][l16];
-        const bool real_site = (c0 + xl) < g.V3;
-        q.x = real_site ? q.x - cr[b] : 0.f;
-        q.y = real_site ? q.y - ci[b] : 0.f;
-        zr[b] += q.x;
-        zi[b] += q.y;
-        _Float16 h, l;
-        split16(q.x * sq, h, l);
-        qrh[v] = h;
-        qrl[v] = l;
-        split16(q.y * sq, h, l);
-        qih[v] = h;
-        qil[v] = l;
-      }
-      A1[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qrh, A1[b], 0, 0, 0);
-      A1[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qrl, A1[b], 0, 0, 0);
-      A1[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, qrh, A1[b], 0, 0, 0);
-      A2[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qih, A2[b], 0, 0, 0);
-      A2[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qil, A2[b], 0, 0, 0);
-      A2[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, qih, A2[b], 0, 0, 0);
-      A3[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qrh, A3[b], 0, 0, 0);
-      A3[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, qrl, A3[b], 0, 0, 0);
-      A3[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, qrh, A3[b], 0, 0, 0);
-      A4[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qih, A4[b], 0, 0, 0);
-      A4[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, qil, A4[b], 0, 0, 0);
-      A4[b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, qih, A4[b], 0, 0, 0);
+      for (int b = 0; b < NBW; ++b) stage2(std::false_type{}, b, c0, sq, ex);
     }
     if (chunk + 1 < nchunk) load_phase(chunk + 1);
-    // FP32 chunk sums into FP64, unscaled exactly (powers of two)
-    const double inv = ldexp(1.0, ex - 28);  // 1 / (2^14 sq)
-#pragma unroll
-    for (int b = 0; b < NBW; ++b) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        D1[b][r] += (double)A1[b][r] * inv;
-        D2[b][r] += (double)A2[b][r] * inv;
-        D3[b][r] += (double)A3[b][r] * inv;
-        D4[b][r] += (double)A4[b][r] * inv;
-      }
-      A1[b] = A2[b] = A3[b] = A4[b] = acc_t{0, 0, 0, 0};
-      Zr[b] += (double)zr[b];
-      Zi[b] += (double)zi[b];
-      zr[b] = zi[b] = 0;
-    }
+    __syncthreads();
   }
 
   // ---- epilogue (as contract_kernel) ----
