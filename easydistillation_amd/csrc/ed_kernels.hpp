@@ -254,8 +254,8 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* _
   const Geom g = a.g;
   const size_t site = (size_t)3 * g.NeP;  // elements per site
   const c_t<T>* Wt = W + (size_t)tb * g.nW * g.V3P * site;
-  const int J = 3 * dp.nsplit;
-  const int nks = (J + 3) >> 2;  // <= NKS
+  const int J = __builtin_amdgcn_readfirstlane(3 * dp.nsplit);
+  const int nks = (J + 3) >> 2;  // <= NKS, uniform
 
   // per k-step operand bases (site 0) and coefficients; j >= J reads a valid
   // element with coefficient 0
@@ -326,52 +326,76 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* _
 #pragma unroll
   for (int b = 0; b < NBW; ++b) cr[b] = ci[b] = 0;
 
-  load_ops(0);
-  load_phase(0);
-  int buf = 0;
-  for (int c0 = 0; c0 < g.V3P; c0 += SC, buf ^= 1) {
-    // ---- stage 1: Q_d[e, f, x] for the SPW sites of this wave ----
+  // stage 1 of the chunk at c0 from the prefetched operands into Qs[buf];
+  // the operands of the next chunk are requested right after use
+  auto stage1 = [&](int c0, int buf) {
     acc_t qr[SPW], qi[SPW];
+    const acc_t zero = {0, 0, 0, 0};
+    T lr[SPW], li[SPW];
 #pragma unroll
-    for (int i = 0; i < SPW; ++i) qr[i] = qi[i] = acc_t{0, 0, 0, 0};
+    for (int i = 0; i < SPW; ++i) {
+      lr[i] = cf[0] * lv[0][i].x;
+      li[i] = cf[0] * lv[0][i].y;
+    }
+    // conj(L) R = (lr rr + li ri) + i (lr ri - li rr); dependent MFMAs are
+    // separated by independent ones
 #pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
+    for (int i = 0; i < SPW; ++i) qr[i] = mfma<T>::mac(lr[i], rv[0][i].x, zero);
+#pragma unroll
+    for (int i = 0; i < SPW; ++i) qi[i] = mfma<T>::mac(lr[i], rv[0][i].y, zero);
+#pragma unroll
+    for (int i = 0; i < SPW; ++i) qr[i] = mfma<T>::mac(li[i], rv[0][i].y, qr[i]);
+#pragma unroll
+    for (int i = 0; i < SPW; ++i) qi[i] = mfma<T>::mac(-li[i], rv[0][i].x, qi[i]);
+#pragma unroll
+    for (int ks = 1; ks < NKS; ++ks) {
       if (ks < nks) {
 #pragma unroll
         for (int i = 0; i < SPW; ++i) {
-          // conj(L) R = (lr rr + li ri) + i (lr ri - li rr)
-          const T lr = cf[ks] * lv[ks][i].x, li = cf[ks] * lv[ks][i].y;
-          qr[i] = mfma<T>::mac(lr, rv[ks][i].x, qr[i]);
-          qr[i] = mfma<T>::mac(li, rv[ks][i].y, qr[i]);
-          qi[i] = mfma<T>::mac(lr, rv[ks][i].y, qi[i]);
-          qi[i] = mfma<T>::mac(-li, rv[ks][i].x, qi[i]);
+          lr[i] = cf[ks] * lv[ks][i].x;
+          li[i] = cf[ks] * lv[ks][i].y;
         }
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) qr[i] = mfma<T>::mac(lr[i], rv[ks][i].x, qr[i]);
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) qi[i] = mfma<T>::mac(lr[i], rv[ks][i].y, qi[i]);
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) qr[i] = mfma<T>::mac(li[i], rv[ks][i].y, qr[i]);
+#pragma unroll
+        for (int i = 0; i < SPW; ++i) qi[i] = mfma<T>::mac(-li[i], rv[ks][i].x, qi[i]);
       }
     }
-    // operands of the next chunk are in flight during stage 2
     if (c0 + SC < g.V3P) load_ops(c0 + SC);
 #pragma unroll
     for (int i = 0; i < SPW; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) Qs[buf][SPW * wave + i][mfma<T>::row(g4, r)][l16] = mk<T>(qr[i][r], qi[i][r]);
-    __syncthreads();
+  };
 
-    if constexpr (sizeof(T) == 4) {
-      if (c0 == 0) {
-        const int n0 = min(SC, g.V3);
+  // Software pipeline: the barrier interval of chunk k holds stage 2 of
+  // chunk k (reading Qs[k & 1]) and stage 1 of chunk k + 1 (writing
+  // Qs[(k + 1) & 1]).
+  load_ops(0);
+  load_phase(0);
+  stage1(0, 0);
+  __syncthreads();
+  if constexpr (sizeof(T) == 4) {
+    const int n0 = min(SC, g.V3);
 #pragma unroll
-        for (int b = 0; b < NBW; ++b) {
-          T sr = 0, si = 0;
-          for (int xl = 0; xl < n0; ++xl) {
-            const c_t<T> q = Qs[buf][xl][NBW * wave + b][l16];
-            sr += q.x;
-            si += q.y;
-          }
-          cr[b] = sr / (T)n0;
-          ci[b] = si / (T)n0;
-        }
+    for (int b = 0; b < NBW; ++b) {
+      T sr = 0, si = 0;
+      for (int xl = 0; xl < n0; ++xl) {
+        const c_t<T> q = Qs[0][xl][NBW * wave + b][l16];
+        sr += q.x;
+        si += q.y;
       }
+      cr[b] = sr / (T)n0;
+      ci[b] = si / (T)n0;
     }
+  }
+  for (int c0 = 0; c0 < g.V3P; c0 += SC) {
+    const int buf = (c0 / SC) & 1;
+    if (c0 + SC < g.V3P) stage1(c0 + SC, buf ^ 1);
 
     // ---- stage 2: phase contraction over the 16 sites of the chunk ----
 #pragma unroll
@@ -412,6 +436,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* _
         zr[b] = zi[b] = 0;
       }
     }
+    __syncthreads();
   }
   if constexpr (sizeof(T) == 8) {
 #pragma unroll
