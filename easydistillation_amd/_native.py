@@ -41,6 +41,9 @@ SIGNATURES = {
     "ed_profile_read": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "ed_profile_reset": (_I, [_VP]),
     "ed_query": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_int64)]),
+    "ed_project_su3": (_I, [_VP, _I, ctypes.POINTER(_I)]),
+    "ed_stout_smear": (_I, [_VP, _I, ctypes.c_double]),
+    "ed_get_gauge": (_I, [_VP, _VP, _I]),
 }
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
@@ -155,3 +158,14 @@ class Context:
 
     def profile_reset(self) -> None:
         self._chk("ed_profile_reset", self.lib.ed_profile_reset(self.h))
+
+    def project_su3(self, max_iter: int = 100) -> int:
+        it = _I()
+        self._chk("ed_project_su3", self.lib.ed_project_su3(self.h, int(max_iter), ctypes.byref(it)))
+        return it.value
+
+    def stout_smear(self, nstep: int, rho: float) -> None:
+        self._chk("ed_stout_smear", self.lib.ed_stout_smear(self.h, int(nstep), float(rho)))
+
+    def get_gauge(self, ptr: int, on_device: bool) -> None:
+        self._chk("ed_get_gauge", self.lib.ed_get_gauge(self.h, _VP(ptr), int(on_device)))

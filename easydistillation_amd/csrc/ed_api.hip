@@ -14,6 +14,7 @@
 
 #include "../../include/elemental_hip.h"
 #include "ed_kernels.hpp"
+#include "ed_gauge.hpp"
 
 using ed::ContractArgs;
 using ed::DPlan;
@@ -807,6 +808,71 @@ int ed_profile_reset(ed_context* ctx) {
       ctx->prof_ms[k] = 0;
       ctx->prof_n[k] = 0;
     }
+  });
+}
+
+int ed_project_su3(ed_context* ctx, int max_iter, int* iterations) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->gauge_set, ED_ERR_STATE, "ed_set_gauge must be called first");
+    if (max_iter < 0) fail(ED_ERR_ARG, "max_iter must be >= 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t nl = (size_t)3 * ctx->Lt * ctx->V3;
+    DevBuf<double2> nxt;
+    nxt.ensure(9 * nl);
+    DevBuf<int> flag;
+    flag.ensure(1);
+    const unsigned blocks = (unsigned)((nl + 255) / 256);
+    int it = 0;
+    for (;; ++it) {
+      HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+      ed::gauge::polar_step_kernel<<<blocks, 256, 0, ctx->stream>>>(ctx->U64.p, nxt.p, nl, 1e-15, flag.p);
+      HIPCHK(hipGetLastError());
+      int h = 0;
+      HIPCHK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      if (!h) break;  // every link passed the test: U is the result
+      if (it == max_iter) fail(ED_ERR_UNSUPPORTED, "unitary projection did not converge in %d iterations", max_iter);
+      std::swap(ctx->U64.p, nxt.p);
+      std::swap(ctx->U64.n, nxt.n);
+    }
+    if (iterations) *iterations = it;
+  });
+}
+
+int ed_stout_smear(ed_context* ctx, int nstep, double rho) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->gauge_set, ED_ERR_STATE, "ed_set_gauge must be called first");
+    if (nstep < 0) fail(ED_ERR_ARG, "nstep must be >= 0");
+    if (!std::isfinite(rho)) fail(ED_ERR_ARG, "rho must be finite");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (nstep == 0) return;
+    const size_t nl = (size_t)3 * ctx->Lt * ctx->V3;
+    DevBuf<double2> nxt;
+    nxt.ensure(9 * nl);
+    const ed::gauge::Lat L{ctx->Lx, ctx->Ly, ctx->Lz, ctx->Lt};
+    const unsigned blocks = (unsigned)((nl + 255) / 256);
+    for (int s = 0; s < nstep; ++s) {
+      ed::gauge::stout_step_kernel<<<blocks, 256, 0, ctx->stream>>>(ctx->U64.p, nxt.p, L, rho);
+      HIPCHK(hipGetLastError());
+      std::swap(ctx->U64.p, nxt.p);
+      std::swap(ctx->U64.n, nxt.n);
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ed_get_gauge(ed_context* ctx, void* U, int on_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->gauge_set, ED_ERR_STATE, "ed_set_gauge must be called first");
+    if (!U) fail(ED_ERR_ARG, "U is null");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)3 * ctx->Lt * ctx->V3 * 9;
+    HIPCHK(hipMemcpyAsync(U, ctx->U64.p, n * sizeof(double2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 

@@ -171,6 +171,44 @@ class ElementalGenerator:
         self._loaded = True
         self._all = None
 
+    # ---------------------------------------------------------- link smearing
+    def project_SU3(self, max_iter: int = 100) -> int:
+        """Unitary projection of the loaded links in device memory (elemental.py:107-117).
+
+        Polar iteration U <- (U + (U^-1)^dagger)/2 until every link passes the
+        reference's 1e-15 tests; returns the number of updates.  The reference
+        loops without bound; here more than ``max_iter`` updates raise.
+        """
+        if not self._loaded:
+            raise RuntimeError("call load(key) or load_arrays(U, V) first")
+        self._all = None
+        return self._ctx.project_su3(max_iter)
+
+    def stout_smear(self, nstep: int, rho: float) -> None:
+        """``nstep`` 3-D stout-smearing steps of weight ``rho`` on the loaded links
+        (elemental.py:264-277, numpy form :175-242), computed on the device."""
+        if not self._loaded:
+            raise RuntimeError("call load(key) or load_arrays(U, V) first")
+        self._all = None
+        self._ctx.stout_smear(nstep, rho)
+
+    def gauge(self):
+        """Current links U[3, Lt, Lz, Ly, Lx, 3, 3] complex128 (numpy, or a device
+        tensor with the "torch" backend)."""
+        if not self._loaded:
+            raise RuntimeError("call load(key) or load_arrays(U, V) first")
+        Lx, Ly, Lz, Lt = self.latt_size
+        shape = (Nd - 1, Lt, Lz, Ly, Lx, Nc, Nc)
+        if get_backend() == "torch":
+            import torch
+
+            out = torch.empty(shape, dtype=torch.complex128, device=f"cuda:{self._ctx.device}")
+            self._ctx.get_gauge(out.data_ptr(), True)
+            return out
+        out = np.empty(shape, dtype=np.complex128)
+        self._ctx.get_gauge(out.ctypes.data, False)
+        return out
+
     # ---------------------------------------------------------------- compute
     def _mode_code(self, mode):
         return _MODES[mode or self.mode]

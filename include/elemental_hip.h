@@ -159,6 +159,30 @@ enum ed_query {
 };
 int ed_query(ed_context* ctx, int what, int64_t* value);
 
+/* Link preprocessing on the context's resident spatial links (run after
+ * ed_set_gauge, before ed_compute; the links are replaced in place).
+ *
+ * ed_project_su3 replaces ElementalGenerator.project_SU3
+ * (lattice/generator/elemental.py:107-117): the polar iteration
+ * U <- (U + (U^-1)^dagger)/2, applied to every link until
+ * max|U - (U^-1)^dagger| <= 1e-15 and max|U U^dagger - 1| <= 1e-15 hold for all
+ * links (the reference's whole-array test).  *iterations (may be NULL)
+ * receives the number of updates; more than max_iter updates fail with
+ * ED_ERR_UNSUPPORTED, leaving the last iterate in place.
+ *
+ * ed_stout_smear replaces ElementalGenerator.stout_smear(nstep, rho)
+ * (elemental.py:175-242,264-277): nstep 3-D stout steps with staples over the two
+ * other spatial directions in the link's own timeslice,
+ * U <- exp(iQ) U, Q = (i/2)(Omega^dagger - Omega) made traceless,
+ * Omega = rho C U^dagger.  Where the reference divides 0/0 (Q = 0) the
+ * identity limit exp(i0) = 1 is used.
+ *
+ * ed_get_gauge copies the current links out as complex128
+ * U[3][Lt][Lz][Ly][Lx][3][3] (host or device pointer). */
+int ed_project_su3(ed_context* ctx, int max_iter, int* iterations);
+int ed_stout_smear(ed_context* ctx, int nstep, double rho);
+int ed_get_gauge(ed_context* ctx, void* U, int on_device);
+
 /* Library version string (static storage). */
 const char* ed_version(void);
 
