@@ -18,6 +18,7 @@ ED_OK = 0
 ED_C128, ED_C64 = 0, 1
 ED_MODE_FP64, ED_MODE_MIXED = 0, 1
 ED_OPT_HERMITIAN, ED_OPT_TBATCH, ED_OPT_WS_MB, ED_OPT_PROFILE, ED_OPT_MIXED_STAGE2 = 1, 2, 3, 4, 5
+ED_OPT_EXACT_V = 6
 ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE = 0, 1, 2
 
 # Every exported symbol and its ctypes signature (restype, argtypes).
@@ -44,6 +45,7 @@ SIGNATURES = {
     "ed_project_su3": (_I, [_VP, _I, ctypes.POINTER(_I)]),
     "ed_stout_smear": (_I, [_VP, _I, ctypes.c_double]),
     "ed_get_gauge": (_I, [_VP, _VP, _I]),
+    "ed_backward": (_I, [_VP, _I, _I, _VP, _I, _VP, _VP, _I]),
 }
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
@@ -169,3 +171,11 @@ class Context:
 
     def get_gauge(self, ptr: int, on_device: bool) -> None:
         self._chk("ed_get_gauge", self.lib.ed_get_gauge(self.h, _VP(ptr), int(on_device)))
+
+    def backward(self, t0: int, nt: int, grad_ptr: int, grad_on_device: bool, gv_ptr, gu_ptr, out_on_device: bool):
+        self._chk(
+            "ed_backward",
+            self.lib.ed_backward(
+                self.h, t0, nt, _VP(grad_ptr), int(grad_on_device), _VP(gv_ptr or 0), _VP(gu_ptr or 0), int(out_on_device)
+            ),
+        )

@@ -15,6 +15,7 @@
 #include "../../include/elemental_hip.h"
 #include "ed_kernels.hpp"
 #include "ed_gauge.hpp"
+#include "ed_adjoint.hpp"
 
 using ed::ContractArgs;
 using ed::DPlan;
@@ -138,6 +139,21 @@ struct ed_context {
   int mixed_stage2 = -1;   // 1: FP16 three-term split, 0: FP32 MFMA, -1: FP16 from V3 >= 4096 on
   int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
 
+  // reverse pass
+  int bns_max = 1;        // largest merged split count of one derivative
+  int nparent_all = 0;    // slots that have children (levels below num_nabla)
+  DevBuf<SplitTerm> bsplits;
+  DevBuf<int2> bdsplit;
+  DevBuf<double2> phase_all;  // [V3][P]
+  DevBuf<int> child_of;       // [nparent_all][3]
+  DevBuf<double2> bW, bgW, bA, bG, bOutV, bOutU;
+  bool bW_clear = false;
+
+  // exact (complex128) eigenvectors for the FP64 form and the reverse pass
+  int exact_v = 0;
+  DevBuf<double2> V64;
+  bool v64_set = false;
+
   // workspace
   int tbatch_opt = 0;
   int64_t ws_mb = 16384;
@@ -222,6 +238,35 @@ void require(bool c, int code, const char* msg) {
 // ---------------------------------------------------------------------------
 // planning
 // ---------------------------------------------------------------------------
+// Split terms of elemental.py:309-321 for the tuple t: bit j of the mask sends
+// t[j] to the right vector, the left list is applied reversed, coefficient
+// (-1)^{#right}; identical (left, right) pairs are merged and cancelled ones
+// dropped.
+std::vector<SplitTerm> merged_splits(const std::vector<int>& t) {
+  std::map<std::pair<int, int>, float> acc;
+  std::vector<std::pair<int, int>> order;
+  const int k = (int)t.size();
+  for (int pick = 0; pick < (1 << k); ++pick) {
+    std::vector<int> right, left;
+    for (int j = 0; j < k; ++j) ((pick >> j) & 1 ? right : left).push_back(t[j]);
+    std::reverse(left.begin(), left.end());
+    const std::pair<int, int> key{derivative_index(left), derivative_index(right)};
+    const float cf = (right.size() % 2) ? -1.f : 1.f;
+    if (!acc.count(key)) order.push_back(key);
+    acc[key] += cf;
+  }
+  std::vector<SplitTerm> out;
+  for (auto& key : order) {
+    if (acc[key] == 0.f) continue;
+    SplitTerm st{};
+    st.nL = key.first;
+    st.nR = key.second;
+    st.coeff = acc[key];
+    out.push_back(st);
+  }
+  return out;
+}
+
 void build_plan(ed_context* c) {
   if (!c->plan_dirty) return;
   require(c->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
@@ -250,27 +295,7 @@ void build_plan(ed_context* c) {
     p.partner = partner;
     p.sign = (t.size() % 2) ? -1 : 1;
     p.first = (int)sp.size();
-    // splits of elemental.py:309-321, identical (left, right) pairs merged
-    std::map<std::pair<int, int>, float> acc;
-    std::vector<std::pair<int, int>> order;
-    const int k = (int)t.size();
-    for (int pick = 0; pick < (1 << k); ++pick) {
-      std::vector<int> right, left;
-      for (int j = 0; j < k; ++j) ((pick >> j) & 1 ? right : left).push_back(t[j]);
-      std::reverse(left.begin(), left.end());
-      const std::pair<int, int> key{derivative_index(left), derivative_index(right)};
-      const float cf = (right.size() % 2) ? -1.f : 1.f;
-      if (!acc.count(key)) order.push_back(key);
-      acc[key] += cf;
-    }
-    for (auto& key : order) {
-      if (acc[key] == 0.f) continue;
-      SplitTerm s{};
-      s.nL = key.first;
-      s.nR = key.second;
-      s.coeff = acc[key];
-      sp.push_back(s);
-    }
+    for (const SplitTerm& st : merged_splits(t)) sp.push_back(st);
     p.nsplit = (int)sp.size() - p.first;
     c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4 * ((3 * p.nsplit + 3) / 4);
     c->nks_max = std::max(c->nks_max, (3 * p.nsplit + 3) / 4);
@@ -383,6 +408,51 @@ void build_plan(ed_context* c) {
   HIPCHK(hipMemcpy(c->ph32.p, pr32.data(), pr32.size() * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->ph32.p + pr32.size(), pi32.data(), pi32.size() * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->ph16.p, p16.data(), p16.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+
+  // reverse pass: merged splits of every derivative (no Hermitian pairing),
+  // Phi_p(x) of every listed momentum as [V3][P], and the child slot of every
+  // (parent slot, direction)
+  {
+    std::vector<SplitTerm> bsp;
+    std::vector<int2> bds(D);
+    c->bns_max = 1;
+    for (int d = 0; d < D; ++d) {
+      const std::vector<SplitTerm> terms = merged_splits(derivative_tuple(d));
+      bds[d] = make_int2((int)bsp.size(), (int)terms.size());
+      bsp.insert(bsp.end(), terms.begin(), terms.end());
+      c->bns_max = std::max(c->bns_max, (int)terms.size());
+    }
+    std::vector<double2> pa((size_t)c->V3 * P);
+    for (int z = 0; z < c->Lz; ++z)
+      for (int y = 0; y < c->Ly; ++y)
+        for (int x = 0; x < c->Lx; ++x) {
+          const size_t st = ((size_t)z * c->Ly + y) * c->Lx + x;
+          for (int p = 0; p < P; ++p) {
+            auto md = [](long a, long m) { long r = a % m; return r < 0 ? r + m : r; };
+            const double fr = (double)md((long)c->moms[3 * p] * x, c->Lx) / c->Lx +
+                              (double)md((long)c->moms[3 * p + 1] * y, c->Ly) / c->Ly +
+                              (double)md((long)c->moms[3 * p + 2] * z, c->Lz) / c->Lz;
+            const double ang = 2.0 * M_PI * fr;
+            pa[st * P + p] = make_double2(std::cos(ang), std::sin(ang));
+          }
+        }
+    c->nparent_all = level_start(c->num_nabla);
+    std::vector<int> child(3 * std::max(1, c->nparent_all), 0);
+    for (int pnt = 0; pnt < c->nparent_all; ++pnt)
+      for (int mu = 0; mu < 3; ++mu) {
+        std::vector<int> t = derivative_tuple(pnt);
+        t.push_back(mu);
+        child[3 * pnt + mu] = derivative_index(t);
+      }
+    c->bsplits.ensure(bsp.size());
+    c->bdsplit.ensure(bds.size());
+    c->phase_all.ensure(pa.size());
+    c->child_of.ensure(child.size());
+    HIPCHK(hipMemcpy(c->bsplits.p, bsp.data(), bsp.size() * sizeof(SplitTerm), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bdsplit.p, bds.data(), bds.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->phase_all.p, pa.data(), pa.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->child_of.p, child.data(), child.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   c->plan_dirty = false;
 }
 
@@ -438,7 +508,10 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     // V(t) -> W slot 0
     c->launch(ED_KERNEL_STAGE, [&] {
       dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
-      ed::stage_v_kernel<T><<<grid, 256, 0, c->stream>>>(c->V.p, W, g, tb0);
+      if (sizeof(T) == 8 && c->exact_v && c->v64_set)
+        ed::stage_v_kernel<T, double2><<<grid, 256, 0, c->stream>>>(c->V64.p, W, g, tb0);
+      else
+        ed::stage_v_kernel<T, float2><<<grid, 256, 0, c->stream>>>(c->V.p, W, g, tb0);
     });
     // covariant hops level by level
     for (int k = 1; k <= c->num_nabla; ++k) {
@@ -508,6 +581,125 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
   if (c->profile) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->drain_events();
+  }
+}
+
+// Reverse pass for timeslices t0 .. t0+nt-1 (FP64).  G [nt][D][P][Ne][Ne];
+// gV [nt][Ne][V3][3] and gU [3][nt][V3][3][3] (either may be null).
+void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, double2* gV, double2* gU, bool o_dev) {
+  const Geom g = c->geom(c->D);
+  const int D = c->D, Ne = c->Ne, P = (int)(c->moms.size() / 3);
+  const size_t wslab = (size_t)D * c->V3P * 3 * c->NeP;
+  const size_t gmat = (size_t)D * P * Ne * Ne;
+  const size_t vt = (size_t)Ne * 3 * c->V3, ut = (size_t)27 * c->V3;
+  size_t per_t = (2 * wslab + 2 * gmat) * sizeof(double2);
+  if (!g_dev) per_t += gmat * sizeof(double2);
+  if (!o_dev) per_t += ((gV ? vt : 0) + (gU ? ut : 0)) * sizeof(double2);
+  int TB = c->tbatch_opt;
+  if (TB <= 0) {
+    const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    TB = (int)std::max<size_t>(1, budget / per_t);
+  }
+  TB = std::min(TB, nt);
+  double2* old = c->bW.p;
+  c->bW.ensure(wslab * TB);
+  if (c->bW.p != old || !c->bW_clear) {
+    // padded rows and sites stay zero: staging and hops write only real ones
+    HIPCHK(hipMemsetAsync(c->bW.p, 0, c->bW.n * sizeof(double2), c->stream));
+    c->bW_clear = true;
+  }
+  c->bgW.ensure(wslab * TB);
+  c->bA.ensure(2 * gmat * TB);
+  if (!g_dev) c->bG.ensure(gmat * TB);
+  if (!o_dev && gV) c->bOutV.ensure(vt * TB);
+  if (!o_dev && gU) c->bOutU.ensure(ut * TB);
+  double2* Arow = c->bA.p;
+  double2* Acol = c->bA.p + gmat * TB;
+  const double2* U = c->U64.p;
+
+  for (int tb0 = t0; tb0 < t0 + nt; tb0 += TB) {
+    const int nb = std::min(TB, t0 + nt - tb0);
+    const int tl = tb0 - t0;
+    // recompute the displaced vectors in FP64
+    {
+      dim3 grid((3 * c->V3 + 63) / 64, (Ne + 63) / 64, nb);
+      if (c->exact_v && c->v64_set)
+        ed::stage_v_kernel<double, double2><<<grid, 256, 0, c->stream>>>(c->V64.p, c->bW.p, g, tb0);
+      else
+        ed::stage_v_kernel<double, float2><<<grid, 256, 0, c->stream>>>(c->V.p, c->bW.p, g, tb0);
+      HIPCHK(hipGetLastError());
+    }
+    for (int k = 1; k <= c->num_nabla; ++k) {
+      const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
+      dim3 grid((c->V3 + 3) / 4, 3 * nparent, nb);
+      ed::hop_kernel<double><<<grid, 256, 0, c->stream>>>(c->bW.p, U, g, tb0, parent0, level_start(k));
+      HIPCHK(hipGetLastError());
+    }
+    // weighted cotangent in row and column order
+    const double2* Gb = G + (size_t)tl * gmat;
+    if (!g_dev) {
+      HIPCHK(hipMemcpyAsync(c->bG.p, Gb, (size_t)nb * gmat * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+      Gb = c->bG.p;
+    }
+    const size_t nmat = (size_t)nb * D * P, ne2 = (size_t)Ne * Ne;
+    for (size_t m0 = 0; m0 < nmat; m0 += 65535) {
+      const unsigned cnt = (unsigned)std::min<size_t>(65535, nmat - m0);
+      dim3 grid((Ne + 31) / 32, (Ne + 31) / 32, cnt);
+      ed::adj::adj_prep_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(Gb + m0 * ne2, c->use_wgt ? c->wgt.p : nullptr,
+                                                                     Arow + m0 * ne2, Acol + m0 * ne2, Ne);
+      HIPCHK(hipGetLastError());
+    }
+    // contraction adjoint
+    HIPCHK(hipMemsetAsync(c->bgW.p, 0, (size_t)nb * wslab * sizeof(double2), c->stream));
+    ed::adj::AdjArgs a{};
+    a.g = g;
+    a.D = D;
+    a.P = P;
+    a.splits = c->bsplits.p;
+    a.dsplit = c->bdsplit.p;
+    a.phase = c->phase_all.p;
+    {
+      dim3 grid((c->V3 + 7) / 8, (Ne + 63) / 64, nb);
+      if (c->bns_max <= 1)
+        ed::adj::adj_contract_kernel<1><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+      else if (c->bns_max <= 2)
+        ed::adj::adj_contract_kernel<2><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+      else if (c->bns_max <= 4)
+        ed::adj::adj_contract_kernel<4><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+      else if (c->bns_max <= 8)
+        ed::adj::adj_contract_kernel<8><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+      else
+        fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", c->bns_max);
+      HIPCHK(hipGetLastError());
+    }
+    // hop adjoint, deepest level first
+    for (int k = c->num_nabla; k >= 1; --k) {
+      const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
+      dim3 grid((c->V3 + 3) / 4, nparent, nb);
+      ed::adj::adj_hop_kernel<<<grid, 256, 0, c->stream>>>(c->bgW.p, U, g, tb0, parent0, level_start(k));
+      HIPCHK(hipGetLastError());
+    }
+    if (gU) {
+      double2* o = o_dev ? gU : c->bOutU.p;
+      dim3 grid((c->V3 + 3) / 4, 3, nb);
+      ed::adj::adj_link_kernel<<<grid, 256, 0, c->stream>>>(c->bW.p, c->bgW.p, c->child_of.p, c->nparent_all, o, g,
+                                                             o_dev ? tl : 0, o_dev ? nt : nb);
+      HIPCHK(hipGetLastError());
+      if (!o_dev)
+        for (int mu = 0; mu < 3; ++mu)
+          HIPCHK(hipMemcpyAsync(gU + ((size_t)mu * nt + tl) * ut / 3, c->bOutU.p + (size_t)mu * nb * ut / 3,
+                                (size_t)nb * ut / 3 * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (gV) {
+      double2* o = o_dev ? gV : c->bOutV.p;
+      dim3 grid((3 * c->V3 + 31) / 32, (Ne + 31) / 32, nb);
+      ed::adj::adj_out_v_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(c->bgW.p, o, g, o_dev ? tl : 0);
+      HIPCHK(hipGetLastError());
+      if (!o_dev)
+        HIPCHK(hipMemcpyAsync(gV + (size_t)tl * vt, c->bOutV.p, (size_t)nb * vt * sizeof(double2),
+                              hipMemcpyDeviceToHost, c->stream));
+    }
+    if (!o_dev || !g_dev) HIPCHK(hipStreamSynchronize(c->stream));
   }
 }
 
@@ -589,12 +781,14 @@ int ed_set_lattice(ed_context* ctx, int Lx, int Ly, int Lz, int Lt, int Ne) {
     ctx->V3P = (int)((v3 + ed::SC2 - 1) / ed::SC2 * ed::SC2);  // multiple of both chunk sizes
     ctx->NeP = (Ne + 15) / 16 * 16;
     ctx->lattice_set = true;
-    ctx->gauge_set = ctx->eig_set = false;
+    ctx->gauge_set = ctx->eig_set = ctx->v64_set = false;
     ctx->use_wgt = false;
     ctx->plan_dirty = true;
     ctx->W.reset();
     ctx->W_elem = 0;
     ctx->stage.reset();
+    ctx->bW.reset();
+    ctx->bW_clear = false;
   });
 }
 
@@ -649,6 +843,20 @@ int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device) {
         HIPCHK(hipGetLastError());
       }
     }
+    ctx->v64_set = false;
+    if (ctx->exact_v) {
+      ctx->V64.ensure(n);
+      if (dtype == ED_C128) {
+        HIPCHK(hipMemcpyAsync(ctx->V64.p, V, n * sizeof(double2),
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+      } else {
+        c64_to_c128_kernel<<<1024, 256, 0, ctx->stream>>>(ctx->V.p, ctx->V64.p, n);
+        HIPCHK(hipGetLastError());
+      }
+      ctx->v64_set = true;
+    } else {
+      ctx->V64.reset();
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->eig_set = true;
   });
@@ -675,6 +883,8 @@ int ed_set_derivatives(ed_context* ctx, int num_nabla) {
     ctx->W.reset();
     ctx->W_elem = 0;
     ctx->stage.reset();
+    ctx->bW.reset();
+    ctx->bW_clear = false;
   });
 }
 
@@ -713,6 +923,9 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
       case ED_OPT_PROFILE:
         ctx->profile = value ? 1 : 0;
         break;
+      case ED_OPT_EXACT_V:
+        ctx->exact_v = value ? 1 : 0;
+        break;
       case ED_OPT_MIXED_STAGE2:
         if (value < -1 || value > 1) fail(ED_ERR_ARG, "mixed stage-2 kind must be -1 (auto), 0 (FP32) or 1 (FP16 split)");
         ctx->mixed_stage2 = (int)value;
@@ -737,6 +950,22 @@ int ed_compute(ed_context* ctx, int t0, int nt, int mode, void* out, int out_on_
       run_batches<double>(ctx, t0, nt, out_on_device ? (double2*)out : nullptr, out_on_device ? nullptr : out);
     else
       run_batches<float>(ctx, t0, nt, out_on_device ? (double2*)out : nullptr, out_on_device ? nullptr : out);
+  });
+}
+
+int ed_backward(ed_context* ctx, int t0, int nt, const void* grad_E, int grad_on_device, void* grad_V, void* grad_U,
+                int out_on_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set && ctx->gauge_set && ctx->eig_set && ctx->mom_set, ED_ERR_STATE,
+            "lattice, gauge, eigenvectors and momenta must be set before ed_backward");
+    if (!grad_E) fail(ED_ERR_ARG, "grad_E is null");
+    if (!grad_V && !grad_U) fail(ED_ERR_ARG, "grad_V and grad_U are both null");
+    if (t0 < 0 || nt < 1 || t0 + nt > ctx->Lt) fail(ED_ERR_ARG, "timeslice range [%d, %d) outside [0, %d)", t0, t0 + nt, ctx->Lt);
+    HIPCHK(hipSetDevice(ctx->device));
+    build_plan(ctx);
+    run_backward(ctx, t0, nt, (const double2*)grad_E, grad_on_device != 0, (double2*)grad_V, (double2*)grad_U,
+                 out_on_device != 0);
   });
 }
 

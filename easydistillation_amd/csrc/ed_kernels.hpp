@@ -7,7 +7,7 @@
 //      order).  Site-major with the eigenvector index fastest: the 16
 //      eigenvectors of a contraction tile at one (site, colour) are one
 //      contiguous segment, and the hop kernel maps lanes to eigenvectors.
-//      NeP = Ne rounded up to 16, V3P = V3 rounded up to 16; padded rows and
+//      NeP = Ne rounded up to 16, V3P = V3 rounded up to 32; padded rows and
 //      sites hold zeros so the contraction needs no tail masks.
 //   U  [3][Lt][V3][3][3]      complex<double> spatial links (elemental.py:103)
 //   Vd [Lt][Ne][V3][3]        complex<float>: the reference stages V in a
@@ -74,23 +74,27 @@ __device__ __forceinline__ size_t widx(const Geom& g, int n, int x, int c, int e
 }
 
 // --------------------------------------------------------------------------
-// Stage V(t) (complex64, [Ne][V3*3]) into slot 0 of W ([V3P*3][NeP]) in the
-// compute precision: a 64 x 64 tile transpose through LDS so that both the
+// Stage V(t) ([Ne][V3*3], complex64 or, for exact eigenvectors, complex128)
+// into slot 0 of W ([V3P*3][NeP]) in the compute precision: a 64 x 64 tile transpose through LDS so that both the
 // read (along sites) and the write (along eigenvectors) are contiguous.
 // grid (ceil(3 V3 / 64), ceil(Ne / 64), TB), block 256
 // --------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void stage_v_kernel(const float2* __restrict__ V, c_t<T>* __restrict__ W,
+template <typename T, typename VT>
+__global__ __launch_bounds__(256) void stage_v_kernel(const VT* __restrict__ V, c_t<T>* __restrict__ W,
                                                        Geom g, int tb0) {
-  __shared__ float2 tile[64][65];
+  __shared__ VT tile[64][65];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int m0 = blockIdx.x * 64, e0 = blockIdx.y * 64, tb = blockIdx.z;
   const int M = 3 * g.V3;
-  const float2* Vt = V + (size_t)(tb0 + tb) * g.Ne * (size_t)M;
+  const VT* Vt = V + (size_t)(tb0 + tb) * g.Ne * (size_t)M;
 #pragma unroll 4
   for (int k = 0; k < 16; ++k) {
     const int el = ty * 16 + k, e = e0 + el, m = m0 + tx;
-    tile[el][tx] = (e < g.Ne && m < M) ? Vt[(size_t)e * M + m] : make_float2(0.f, 0.f);
+    VT v;
+    v.x = 0;
+    v.y = 0;
+    if (e < g.Ne && m < M) v = Vt[(size_t)e * M + m];
+    tile[el][tx] = v;
   }
   __syncthreads();
   c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;  // slot 0
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(256) void stage_v_kernel(const float2* __restrict__
   for (int k = 0; k < 16; ++k) {
     const int ml = ty * 16 + k, m = m0 + ml, e = e0 + tx;
     if (m < M && e < g.Ne) {
-      const float2 v = tile[tx][ml];
+      const VT v = tile[tx][ml];
       Wt[(size_t)m * g.NeP + e] = mk<T>((T)v.x, (T)v.y);
     }
   }

@@ -114,13 +114,18 @@ int ed_synchronize(ed_context* ctx);
  *   ED_OPT_MIXED_STAGE2   mixed form's phase contraction: 1 FP16 MFMA on a three-term
  *                         hi/lo split, 0 FP32 MFMA, -1 (default) FP16 when V3 >= 4096
  *                         (small lattices keep FP32, whose error on blocks that are small
- *                         against their terms is ~2.5x lower). */
+ *                         against their terms is ~2.5x lower).
+ *   ED_OPT_EXACT_V        1: the next ed_set_eigvecs also keeps V in complex128 and the FP64
+ *                         form and ed_backward use it unrounded (for derivative checks by
+ *                         finite differences); 0 (default): V is rounded to complex64 as
+ *                         in the reference. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
   ED_OPT_WS_MB = 3,
   ED_OPT_PROFILE = 4,
-  ED_OPT_MIXED_STAGE2 = 5
+  ED_OPT_MIXED_STAGE2 = 5,
+  ED_OPT_EXACT_V = 6
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 
@@ -158,6 +163,21 @@ enum ed_query {
   ED_Q_MIXED_F16_FLOPS_PER_T = 8
 };
 int ed_query(ed_context* ctx, int what, int64_t* value);
+
+/* Reverse pass of ed_compute's FP64 form (no counterpart in the reference;
+ * the caller is easydistillation_amd.autograd.ElementalFunction.backward).
+ * Given grad_E = dL/dRe(E) + i dL/dIm(E) for a real loss L, laid out as
+ * ed_compute's out ([nt][D][P][Ne][Ne] complex128), writes
+ *   grad_V [nt][Ne][Lz][Ly][Lx][3]        dL/dRe(V) + i dL/dIm(V)
+ *   grad_U [3][nt][Lz][Ly][Lx][3][3]      dL/dRe(U) + i dL/dIm(U)
+ * for timeslices t0 .. t0+nt-1 (either output may be NULL).  The displaced
+ * vectors are recomputed in FP64, the blending weights of ed_set_weights
+ * apply, and the gradient passes straight through the complex64 rounding of
+ * V unless ED_OPT_EXACT_V is set.  grad_on_device / out_on_device: the
+ * pointers are device memory (the call returns after enqueueing) or host
+ * memory (the call returns when the outputs are filled). */
+int ed_backward(ed_context* ctx, int t0, int nt, const void* grad_E, int grad_on_device, void* grad_V, void* grad_U,
+                int out_on_device);
 
 /* Link preprocessing on the context's resident spatial links (run after
  * ed_set_gauge, before ed_compute; the links are replaced in place).

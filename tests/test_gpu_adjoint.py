@@ -1,0 +1,153 @@
+"""The hand-written HIP adjoint (ed_backward) against FP64 torch autograd and
+central finite differences.
+
+Bars: gradients within 1e-11 (normwise max|d| / max|ref|) of torch autograd
+of the FP64 restatement (oracle/elemental_torch.py); finite differences
+(h = 1e-6, central) agree with the analytic directional derivative within
+1e-6 relative.  Forward and backward run with unrounded V (ED_OPT_EXACT_V).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from easydistillation_amd import _native
+from easydistillation_amd import synthetic as S
+from easydistillation_amd.autograd import ElementalEngine, elemental, profiled_loss
+from oracle import elemental_oracle as O
+from oracle import elemental_torch as T
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-11
+FD_TOL = 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def nrm(a, b):
+    a = a.detach().cpu() if isinstance(a, torch.Tensor) else torch.as_tensor(a)
+    b = b.detach().cpu() if isinstance(b, torch.Tensor) else torch.as_tensor(b)
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
+
+
+def inputs(latt, Ne, seed, kind="haar"):
+    rng = np.random.default_rng(seed)
+    U = S.gauge_np(rng, latt, kind)
+    V = S.eigvecs_np(rng, latt, Ne).astype(np.complex128)
+    return rng, U, V
+
+
+CASES = [
+    # latt, Ne, num_nabla, momenta, blending, tbatch
+    ([4, 5, 6, 2], 6, 0, S.unit_momenta(), False, 0),
+    ([4, 5, 6, 2], 8, 1, S.momenta_by_norm(2), False, 0),
+    ([3, 4, 5, 2], 5, 2, S.unit_momenta(), True, 0),
+    ([4, 4, 4, 3], 7, 2, S.momenta_by_norm(1), False, 1),
+    ([5, 3, 4, 2], 70, 1, [(0, 0, 0), (1, 2, 0), (0, 0, 0), (-1, -2, 0)], False, 0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_backward_matches_torch_autograd(case):
+    latt, Ne, nn, moms, blend, tbatch = CASES[case]
+    say(f"adjoint case {case}: latt={latt} Ne={Ne} nabla={nn} P={len(moms)} blend={blend} tbatch={tbatch}")
+    rng, U, V = inputs(latt, Ne, 40 + case)
+    wgt = rng.uniform(0.5, 1.5, size=(Ne, Ne)) if blend else None
+    D = O.num_derivative(nn)
+    G = rng.normal(size=(latt[3], D, len(moms), Ne, Ne)) + 1j * rng.normal(size=(latt[3], D, len(moms), Ne, Ne))
+
+    Ut = torch.from_numpy(U).requires_grad_()
+    Vt = torch.from_numpy(V).requires_grad_()
+    Eref = T.elemental(Vt, Ut, latt, nn, moms, wgt)
+    L = (torch.from_numpy(G).conj() * Eref).real.sum()
+    gU_ref, gV_ref = torch.autograd.grad(L, [Ut, Vt], allow_unused=True)
+    if gU_ref is None:  # no derivatives: E does not depend on U
+        gU_ref = torch.zeros_like(Ut)
+
+    eng = ElementalEngine(latt, Ne, nn, moms, wgt)
+    if tbatch:
+        eng.ctx.set_option(_native.ED_OPT_TBATCH, tbatch)
+    Ud = torch.from_numpy(U).cuda()
+    Vd = torch.from_numpy(V).cuda()
+    E = eng.forward(Ud, Vd)
+    ef = nrm(E, Eref)
+    gU, gV = eng.backward(Ud, Vd, torch.from_numpy(G).cuda())
+    eu = nrm(gU, gU_ref) if nn else float(gU.abs().max())
+    ev = nrm(gV, gV_ref)
+    say(f"  forward {ef:.2e}  dL/dU {eu:.2e}  dL/dV {ev:.2e}")
+    assert ef < 1e-12 and eu < GRAD_TOL and ev < GRAD_TOL
+
+
+def test_backward_host_pointers():
+    latt, Ne, nn = [4, 4, 5, 2], 6, 1
+    moms = S.unit_momenta()
+    rng, U, V = inputs(latt, Ne, 7)
+    D = O.num_derivative(nn)
+    G = rng.normal(size=(latt[3], D, len(moms), Ne, Ne)) + 1j * rng.normal(size=(latt[3], D, len(moms), Ne, Ne))
+    eng = ElementalEngine(latt, Ne, nn, moms)
+    Ud, Vd = torch.from_numpy(U).cuda(), torch.from_numpy(V).cuda()
+    gU_dev, gV_dev = eng.backward(Ud, Vd, torch.from_numpy(G).cuda())
+    gV = np.empty(V.shape, np.complex128)
+    gU = np.empty(U.shape, np.complex128)
+    # second timeslice only, host in and out
+    Gh = np.ascontiguousarray(G[1:2])
+    gV1 = np.empty((1,) + V.shape[1:], np.complex128)
+    gU1 = np.empty((3, 1) + U.shape[2:], np.complex128)
+    eng.ctx.backward(1, 1, Gh.ctypes.data, False, gV1.ctypes.data, gU1.ctypes.data, False)
+    assert np.abs(gV1[0] - gV_dev[1].cpu().numpy()).max() == 0
+    assert np.abs(gU1[:, 0] - gU_dev[:, 1].cpu().numpy()).max() == 0
+    del gV, gU
+
+
+def test_profiled_loss_gradients_match_autograd():
+    latt, Ne, nn = [4, 5, 4, 2], 9, 1
+    moms = S.unit_momenta()
+    rng, U, V = inputs(latt, Ne, 17)
+    w = rng.uniform(0.5, 1.5, Ne)
+    Ut = torch.from_numpy(U).requires_grad_()
+    Vt = torch.from_numpy(V).requires_grad_()
+    wt = torch.from_numpy(w).requires_grad_()
+    Lref = T.profiled_loss(T.elemental(Vt, Ut, latt, nn, moms), wt)
+    gref = torch.autograd.grad(Lref, [Ut, Vt, wt])
+
+    eng = ElementalEngine(latt, Ne, nn, moms)
+    Ud = torch.from_numpy(U).cuda().requires_grad_()
+    Vd = torch.from_numpy(V).cuda().requires_grad_()
+    wd = torch.from_numpy(w).cuda().requires_grad_()
+    L = profiled_loss(Ud, Vd, wd, eng)
+    g = torch.autograd.grad(L, [Ud, Vd, wd])
+    errs = [nrm(a, b) for a, b in zip(g, gref)]
+    say(f"profiled loss: L {float(L):.6e} ref {float(Lref):.6e}; grad errors U {errs[0]:.2e} V {errs[1]:.2e} w {errs[2]:.2e}")
+    assert abs(float(L) - float(Lref)) < 1e-12 * abs(float(Lref))
+    assert max(errs) < GRAD_TOL
+
+
+def test_finite_differences_64_directions():
+    latt, Ne, nn = [6, 6, 6, 4], 12, 1
+    moms = S.unit_momenta()
+    rng, U, V = inputs(latt, Ne, 23)
+    w = torch.from_numpy(rng.uniform(0.5, 1.5, Ne)).cuda()
+    eng = ElementalEngine(latt, Ne, nn, moms)
+    Ud = torch.from_numpy(U).cuda().requires_grad_()
+    Vd = torch.from_numpy(V).cuda().requires_grad_()
+    gU, gV = torch.autograd.grad(profiled_loss(Ud, Vd, w, eng), [Ud, Vd])
+    h = 1e-6
+    g = torch.Generator(device="cuda").manual_seed(5)
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(64):
+            dU = torch.randn(U.shape, dtype=torch.complex128, device="cuda", generator=g)
+            dV = torch.randn(V.shape, dtype=torch.complex128, device="cuda", generator=g)
+            Lp = float(T.profiled_loss(eng.forward(Ud + h * dU, Vd + h * dV), w))
+            Lm = float(T.profiled_loss(eng.forward(Ud - h * dU, Vd - h * dV), w))
+            fd = (Lp - Lm) / (2 * h)
+            an = float((gU.conj() * dU).real.sum() + (gV.conj() * dV).real.sum())
+            worst = max(worst, abs(fd - an) / abs(an))
+    say(f"FD 64 directions: worst relative difference {worst:.2e}")
+    assert worst < FD_TOL
