@@ -95,12 +95,14 @@ class ElementalEngine:
         self.ctx.set_eigvecs(Vd.data_ptr(), _native.ED_C128, True)
         self._resident = (U, U._version, V, V._version)
 
-    def forward(self, U, V, mode: str = "fp64") -> torch.Tensor:
+    def forward(self, U, V, mode: str = "fp64", t0: int = 0, nt: int | None = None) -> torch.Tensor:
+        """E [nt, D, P, Ne, Ne] for timeslices t0 .. t0+nt-1 (all by default)."""
+        nt = self.Lt - t0 if nt is None else nt
         self.load(U, V)
-        out = torch.empty((self.Lt, self.D, self.P, self.Ne, self.Ne), dtype=torch.complex128, device=self._dev())
+        out = torch.empty((nt, self.D, self.P, self.Ne, self.Ne), dtype=torch.complex128, device=self._dev())
         self._pre()
         code = _native.ED_MODE_FP64 if mode == "fp64" else _native.ED_MODE_MIXED
-        self.ctx.compute(0, self.Lt, code, out.data_ptr(), True)
+        self.ctx.compute(t0, nt, code, out.data_ptr(), True)
         self._post()
         return out
 
