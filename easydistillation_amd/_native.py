@@ -46,7 +46,18 @@ SIGNATURES = {
     "ed_stout_smear": (_I, [_VP, _I, ctypes.c_double]),
     "ed_get_gauge": (_I, [_VP, _VP, _I]),
     "ed_backward": (_I, [_VP, _I, _I, _VP, _I, _VP, _VP, _I]),
+    "ed_batch_create": (_I, [_I, ctypes.POINTER(_VP)]),
+    "ed_batch_destroy": (_I, [_VP]),
+    "ed_batch_last_error": (ctypes.c_char_p, [_VP]),
+    "ed_batch_add": (_I, [_VP, _VP, _I, _VP]),
+    "ed_batch_set_option": (_I, [_VP, _I, ctypes.c_int64]),
+    "ed_batch_get_option": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_int64)]),
+    "ed_batch_run": (_I, [_VP]),
+    "ed_batch_synchronize": (_I, [_VP]),
+    "ed_batch_time": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double)]),
+    "ed_batch_tune": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double)]),
 }
+ED_BOPT_GRID_PER_CU, ED_BOPT_ORDER, ED_BOPT_GRAPH = 1, 2, 3
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
 ED_Q_MIXED_F32_FLOPS_PER_T, ED_Q_MIXED_F16_FLOPS_PER_T = 7, 8
@@ -179,3 +190,61 @@ class Context:
                 self.h, t0, nt, _VP(grad_ptr), int(grad_on_device), _VP(gv_ptr or 0), _VP(gu_ptr or 0), int(out_on_device)
             ),
         )
+
+
+class Batch:
+    """A batch of contexts computed by one graph launch (``ed_batch_*``)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _VP()
+        rc = self.lib.ed_batch_create(device, ctypes.byref(h))
+        if rc != ED_OK:
+            raise NativeError("ed_batch_create", rc, "could not create a batch (is a GPU visible?)")
+        self.h = h
+        self.device = device
+        self._members = []  # keeps the contexts alive while the batch refers to them
+
+    def _chk(self, name: str, rc: int) -> None:
+        if rc != ED_OK:
+            msg = self.lib.ed_batch_last_error(self.h)
+            raise NativeError(name, rc, msg.decode() if msg else "")
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.ed_batch_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, ctx: Context, mode: int, out_ptr: int) -> None:
+        self._chk("ed_batch_add", self.lib.ed_batch_add(self.h, ctx.h, mode, _VP(out_ptr)))
+        self._members.append(ctx)
+
+    def set_option(self, option: int, value: int) -> None:
+        self._chk("ed_batch_set_option", self.lib.ed_batch_set_option(self.h, option, int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        self._chk("ed_batch_get_option", self.lib.ed_batch_get_option(self.h, option, ctypes.byref(v)))
+        return v.value
+
+    def run(self) -> None:
+        self._chk("ed_batch_run", self.lib.ed_batch_run(self.h))
+
+    def synchronize(self) -> None:
+        self._chk("ed_batch_synchronize", self.lib.ed_batch_synchronize(self.h))
+
+    def time(self, reps: int = 1) -> float:
+        ms = ctypes.c_double()
+        self._chk("ed_batch_time", self.lib.ed_batch_time(self.h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def tune(self, reps: int = 1) -> float:
+        ms = ctypes.c_double()
+        self._chk("ed_batch_tune", self.lib.ed_batch_tune(self.h, int(reps), ctypes.byref(ms)))
+        return ms.value

@@ -138,6 +138,7 @@ struct ed_context {
   DevBuf<ed::half8> ph16;  // [npb][V3P/32][4][16][32] fp16 hi/lo of 2^14 Phi
   int mixed_stage2 = -1;   // 1: FP16 three-term split, 0: FP32 MFMA, -1: FP16 from V3 >= 4096 on
   int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
+  uint64_t gen = 0;        // bumped whenever buffers or plan a captured launch refers to change
 
   // reverse pass
   int bns_max = 1;        // largest merged split count of one derivative
@@ -454,6 +455,7 @@ void build_plan(ed_context* c) {
     HIPCHK(hipMemcpy(c->child_of.p, child.data(), child.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   c->plan_dirty = false;
+  ++c->gen;
 }
 
 __global__ void c128_to_c64_kernel(const double2* __restrict__ in, float2* __restrict__ out, size_t n) {
@@ -470,31 +472,91 @@ size_t per_t_ws_bytes(const ed_context* c) {
   return (size_t)c->D * c->NeP * 3 * c->V3P * sizeof(ed::c_t<T>);
 }
 
+// the mixed form takes the FP16-MFMA phase contraction (addressing one
+// timeslice's workspace with 32-bit buffer offsets) when allowed and chosen
+bool use_f16_stage2(const ed_context* c) {
+  const bool fits32 = per_t_ws_bytes<float>(c) < ((size_t)1 << 31);
+  return fits32 && (c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096));
+}
+
+// Workspace for TB timeslices in element type T; padded rows and sites must
+// hold zeros (they are never written), so a new buffer, or one last used
+// with the other element type, is cleared.
+template <typename T>
+ed::c_t<T>* ensure_workspace(ed_context* c, int TB) {
+  const size_t ws = per_t_ws_bytes<T>(c) * TB;
+  if (c->W.n < ws || c->W_elem != (int)sizeof(ed::c_t<T>)) {
+    c->W.ensure(ws);
+    HIPCHK(hipMemsetAsync(c->W.p, 0, c->W.n, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->W_elem = (int)sizeof(ed::c_t<T>);
+    ++c->gen;
+  }
+  return (ed::c_t<T>*)(void*)c->W.p;
+}
+
+template <typename T>
+int auto_tbatch(const ed_context* c, int nt) {
+  int TB = c->tbatch_opt;
+  if (TB <= 0) {
+    const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    TB = (int)std::max<size_t>(1, budget / per_t_ws_bytes<T>(c));
+  }
+  return std::min(TB, nt);
+}
+
+// V(t) -> slot 0, then the covariant hops level by level, for timeslices
+// tb0 .. tb0+nb-1 into W; launch(kind, f) wraps each launch.
+template <typename T, class L>
+void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t st, L&& launch) {
+  const Geom g = c->geom(c->D);
+  launch(ED_KERNEL_STAGE, [&] {
+    dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
+    if (sizeof(T) == 8 && c->exact_v && c->v64_set)
+      ed::stage_v_kernel<T, double2><<<grid, 256, 0, st>>>(c->V64.p, W, g, tb0);
+    else
+      ed::stage_v_kernel<T, float2><<<grid, 256, 0, st>>>(c->V.p, W, g, tb0);
+  });
+  for (int k = 1; k <= c->num_nabla; ++k) {
+    const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
+    const int child0 = level_start(k);
+    launch(ED_KERNEL_STENCIL, [&] {
+      dim3 grid((c->V3 + 3) / 4, 3 * nparent, nb);
+      ed::hop_kernel<T><<<grid, 256, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0);
+    });
+  }
+}
+
+ContractArgs contract_args(const ed_context* c, int nb, int t_out0) {
+  ContractArgs a{};
+  a.g = c->geom(c->D);
+  a.D = c->D;
+  a.P = (int)(c->moms.size() / 3);
+  a.nd = c->nd;
+  a.npb = c->npb;
+  a.nbase = c->nbase;
+  a.nte = c->NeP / 16;
+  a.ntf = c->NeP / 16;
+  a.nwork = nb * a.nd * a.npb * a.nte * a.ntf;
+  a.zero_idx = c->zero_idx;
+  a.t_out0 = t_out0;
+  a.dplan = c->dplan.p;
+  a.splits = c->splits.p;
+  a.plus_idx = c->pidx.p;
+  a.minus_idx = c->pidx.p + c->npb * 16;
+  a.wgt = c->use_wgt ? c->wgt.p : nullptr;
+  a.psum = c->psum.p;
+  return a;
+}
+
 template <typename T>
 void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host) {
   const T* ph = nullptr;
   if (sizeof(T) == 8) ph = (const T*)(void*)c->ph64.p;
   else ph = (const T*)(void*)c->ph32.p;
   const size_t phn = (size_t)c->npb * 16 * c->V3P;
-  const double2* U = c->U64.p;
-
-  const size_t per_t = per_t_ws_bytes<T>(c);
-  int TB = c->tbatch_opt;
-  if (TB <= 0) {
-    const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
-    TB = (int)std::max<size_t>(1, budget / per_t);
-  }
-  TB = std::min(TB, nt);
-  const size_t ws = per_t * TB;
-  if (c->W.n < ws || c->W_elem != (int)sizeof(ed::c_t<T>)) {
-    c->W.ensure(ws);
-    // padded rows and sites must hold zeros (they are never written); a
-    // buffer last used with the other element type is cleared again
-    HIPCHK(hipMemsetAsync(c->W.p, 0, c->W.n, c->stream));
-    c->W_elem = (int)sizeof(ed::c_t<T>);
-  }
-  ed::c_t<T>* W = (ed::c_t<T>*)(void*)c->W.p;
-  const Geom g = c->geom(c->D);
+  const int TB = auto_tbatch<T>(c, nt);
+  ed::c_t<T>* W = ensure_workspace<T>(c, TB);
   const size_t out_t = (size_t)c->D * (c->moms.size() / 3) * c->Ne * c->Ne;
 
   double2* out = out_dev;
@@ -502,50 +564,14 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     c->stage.ensure(out_t * TB);
     out = c->stage.p;
   }
+  auto launch = [c](int kind, auto&& f) { c->launch(kind, f); };
 
   for (int tb0 = t0; tb0 < t0 + nt; tb0 += TB) {
     const int nb = std::min(TB, t0 + nt - tb0);
-    // V(t) -> W slot 0
-    c->launch(ED_KERNEL_STAGE, [&] {
-      dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
-      if (sizeof(T) == 8 && c->exact_v && c->v64_set)
-        ed::stage_v_kernel<T, double2><<<grid, 256, 0, c->stream>>>(c->V64.p, W, g, tb0);
-      else
-        ed::stage_v_kernel<T, float2><<<grid, 256, 0, c->stream>>>(c->V.p, W, g, tb0);
-    });
-    // covariant hops level by level
-    for (int k = 1; k <= c->num_nabla; ++k) {
-      const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
-      const int child0 = level_start(k);
-      c->launch(ED_KERNEL_STENCIL, [&] {
-        dim3 grid((c->V3 + 3) / 4, 3 * nparent, nb);
-        ed::hop_kernel<T><<<grid, 256, 0, c->stream>>>(W, U, g, tb0, parent0, child0);
-      });
-    }
-    // contraction
-    ContractArgs a{};
-    a.g = g;
-    a.D = c->D;
-    a.P = (int)(c->moms.size() / 3);
-    a.nd = c->nd;
-    a.npb = c->npb;
-    a.nbase = c->nbase;
-    a.nte = c->NeP / 16;
-    a.ntf = c->NeP / 16;
-    a.nwork = nb * a.nd * a.npb * a.nte * a.ntf;
-    a.zero_idx = c->zero_idx;
-    a.t_out0 = out_dev ? (tb0 - t0) : 0;
-    a.dplan = c->dplan.p;
-    a.splits = c->splits.p;
-    a.plus_idx = c->pidx.p;
-    a.minus_idx = c->pidx.p + c->npb * 16;
-    a.wgt = c->use_wgt ? c->wgt.p : nullptr;
-    a.psum = c->psum.p;
+    enqueue_stencil<T>(c, W, tb0, nb, c->stream, launch);
+    ContractArgs a = contract_args(c, nb, out_dev ? (tb0 - t0) : 0);
     const int nblk = (a.nwork + 7) / 8 * 8;
-    // the FP16 kernel addresses one timeslice's workspace with 32-bit buffer offsets
-    const bool fits32 = per_t_ws_bytes<float>(c) < ((size_t)1 << 31);
-    const bool f16_stage2 = fits32 && (c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096));
-    if (sizeof(T) == 4 && f16_stage2) {
+    if (sizeof(T) == 4 && use_f16_stage2(c)) {
       const float2* Wf = (const float2*)(const void*)W;
       c->launch(ED_KERNEL_CONTRACT, [&] {
         if (c->nks_max <= 1)
@@ -621,20 +647,10 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
     const int nb = std::min(TB, t0 + nt - tb0);
     const int tl = tb0 - t0;
     // recompute the displaced vectors in FP64
-    {
-      dim3 grid((3 * c->V3 + 63) / 64, (Ne + 63) / 64, nb);
-      if (c->exact_v && c->v64_set)
-        ed::stage_v_kernel<double, double2><<<grid, 256, 0, c->stream>>>(c->V64.p, c->bW.p, g, tb0);
-      else
-        ed::stage_v_kernel<double, float2><<<grid, 256, 0, c->stream>>>(c->V.p, c->bW.p, g, tb0);
+    enqueue_stencil<double>(c, c->bW.p, tb0, nb, c->stream, [](int, auto&& f) {
+      f();
       HIPCHK(hipGetLastError());
-    }
-    for (int k = 1; k <= c->num_nabla; ++k) {
-      const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
-      dim3 grid((c->V3 + 3) / 4, 3 * nparent, nb);
-      ed::hop_kernel<double><<<grid, 256, 0, c->stream>>>(c->bW.p, U, g, tb0, parent0, level_start(k));
-      HIPCHK(hipGetLastError());
-    }
+    });
     // weighted cotangent in row and column order
     const double2* Gb = G + (size_t)tl * gmat;
     if (!g_dev) {
@@ -722,6 +738,229 @@ int guard(ed_context* ctx, const std::function<void()>& body) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Batched form: several contexts (lattices of different size and Ne) computed
+// by one stream-ordered sequence -- per lattice the staging and hop kernels
+// over all its timeslices, then one persistent grouped contraction per kernel
+// kind over every lattice -- captured once into a hipGraph and replayed with a
+// single launch.
+// ---------------------------------------------------------------------------
+struct ed_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  struct Item {
+    ed_context* ctx;
+    int mode;
+    double2* out;
+  };
+  std::vector<Item> items;
+  int grid_per_cu = 2;  // persistent grid = grid_per_cu x CUs; 0: one workgroup per item
+  int order = 0;        // item order of the grouped kernels
+  int use_graph = 1;
+  int ncu = 256;
+  // kinds: 0 FP64 MFMA, 1 FP32 MFMA (mixed, FP32 stage 2), 2 mixed FP16 stage 2
+  DevBuf<ed::GroupProblem> probs[3];
+  std::vector<ed::GroupProblem> host_probs[3];
+  int nprob[3] = {0, 0, 0}, total[3] = {0, 0, 0}, nks[3] = {1, 1, 1};
+  std::vector<uint64_t> fingerprint;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    fingerprint.clear();
+  }
+  ~ed_batch() {
+    drop_graph();
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+int bguard(ed_batch* b, const std::function<void()>& body) {
+  try {
+    body();
+    if (b) b->err.clear();
+    return ED_OK;
+  } catch (const EdError& e) {
+    if (b) b->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (b) b->err = e.what();
+    return ED_ERR_ALLOC;
+  } catch (...) {
+    if (b) b->err = "unknown error";
+    return ED_ERR_ARG;
+  }
+}
+
+int batch_kind(const ed_context* c, int mode) {
+  if (mode == ED_MODE_FP64) return 0;
+  return use_f16_stage2(c) ? 2 : 1;
+}
+
+// Plans, workspaces (every timeslice at once) and the grouped problem tables;
+// returns the fingerprint of everything a captured graph refers to.
+std::vector<uint64_t> batch_prepare(ed_batch* b) {
+  std::vector<ed::GroupProblem> tab[3];
+  std::vector<uint64_t> fp;
+  for (auto& it : b->items) {
+    ed_context* c = it.ctx;
+    require(c->lattice_set && c->gauge_set && c->eig_set && c->mom_set, ED_ERR_STATE,
+            "every context of a batch needs lattice, gauge, eigenvectors and momenta");
+    HIPCHK(hipSetDevice(c->device));
+    build_plan(c);
+    const int kind = batch_kind(c, it.mode);
+    void* W = kind == 0 ? (void*)ensure_workspace<double>(c, c->Lt) : (void*)ensure_workspace<float>(c, c->Lt);
+    ed::GroupProblem gp{};
+    gp.a = contract_args(c, c->Lt, 0);
+    gp.W = W;
+    if (kind == 0) {
+      const size_t phn = (size_t)c->npb * 16 * c->V3P;
+      gp.ph0 = c->ph64.p;
+      gp.ph1 = c->ph64.p + phn;
+    } else if (kind == 1) {
+      const size_t phn = (size_t)c->npb * 16 * c->V3P;
+      gp.ph0 = c->ph32.p;
+      gp.ph1 = c->ph32.p + phn;
+    } else {
+      gp.ph0 = c->ph16.p;
+      gp.ph1 = nullptr;
+    }
+    gp.out = it.out;
+    gp.work0 = b->total[kind];
+    tab[kind].push_back(gp);
+    b->nks[kind] = std::max(b->nks[kind], c->nks_max);
+    fp.push_back((uint64_t)(uintptr_t)c);
+    fp.push_back(c->gen);
+    fp.push_back((uint64_t)(uintptr_t)W);
+    fp.push_back((uint64_t)(uintptr_t)c->U64.p);
+    fp.push_back((uint64_t)(uintptr_t)c->V.p);
+    fp.push_back((uint64_t)(uintptr_t)c->V64.p);
+    int64_t tot = (int64_t)b->total[kind] + gp.a.nwork;
+    if (tot >= ((int64_t)1 << 31)) fail(ED_ERR_UNSUPPORTED, "batch has too many work items");
+    b->total[kind] = (int)tot;
+  }
+  for (int k = 0; k < 3; ++k) {
+    b->nprob[k] = (int)tab[k].size();
+    if (tab[k].empty()) continue;
+    const size_t bytes = tab[k].size() * sizeof(ed::GroupProblem);
+    ed::GroupProblem* old = b->probs[k].p;
+    b->probs[k].ensure(tab[k].size());
+    // upload only what changed: a replayed graph may still be reading the table
+    if (b->probs[k].p != old || b->host_probs[k].size() != tab[k].size() ||
+        std::memcmp(b->host_probs[k].data(), tab[k].data(), bytes) != 0) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      HIPCHK(hipMemcpy(b->probs[k].p, tab[k].data(), bytes, hipMemcpyHostToDevice));
+      b->host_probs[k] = tab[k];
+    }
+    fp.push_back((uint64_t)(uintptr_t)b->probs[k].p);
+  }
+  fp.push_back((uint64_t)b->grid_per_cu);
+  fp.push_back((uint64_t)b->order);
+  return fp;
+}
+
+int batch_grid(const ed_batch* b, int total) {
+  const int full = (total + 7) / 8 * 8;
+  if (b->grid_per_cu <= 0) return full;
+  const int pers = (b->grid_per_cu * b->ncu + 7) / 8 * 8;
+  return std::min(full, pers);
+}
+
+void batch_enqueue(ed_batch* b) {
+  auto direct = [](int, auto&& f) {
+    f();
+    HIPCHK(hipGetLastError());
+  };
+  for (auto& it : b->items) {
+    ed_context* c = it.ctx;
+    if (batch_kind(c, it.mode) == 0)
+      enqueue_stencil<double>(c, (double2*)(void*)c->W.p, 0, c->Lt, b->stream, direct);
+    else
+      enqueue_stencil<float>(c, (float2*)(void*)c->W.p, 0, c->Lt, b->stream, direct);
+  }
+  const dim3 blk(64 * ed::NWAVE);
+  for (int k = 0; k < 3; ++k) {
+    if (!b->nprob[k]) continue;
+    const int grid = batch_grid(b, b->total[k]);
+    const ed::GroupProblem* P = b->probs[k].p;
+    const int np = b->nprob[k], tot = b->total[k], ord = b->order, nks = b->nks[k];
+    if (nks > 6) fail(ED_ERR_UNSUPPORTED, "derivative with more split terms than the batched kernels take");
+    const int ks = nks <= 1 ? 1 : nks <= 2 ? 2 : nks <= 3 ? 3 : 6;
+#define ED_GROUP(KS)                                                                                    \
+  do {                                                                                                  \
+    if (k == 0) ed::contract_group_kernel<double, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);   \
+    else if (k == 1) ed::contract_group_kernel<float, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord); \
+    else ed::contract_mixed_group_kernel<KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);            \
+  } while (0)
+    if (ks == 1) ED_GROUP(1);
+    else if (ks == 2) ED_GROUP(2);
+    else if (ks == 3) ED_GROUP(3);
+    else ED_GROUP(6);
+#undef ED_GROUP
+    HIPCHK(hipGetLastError());
+  }
+}
+
+void batch_run(ed_batch* b) {
+  require(!b->items.empty(), ED_ERR_STATE, "the batch is empty");
+  HIPCHK(hipSetDevice(b->device));
+  for (int k = 0; k < 3; ++k) {
+    b->total[k] = 0;
+    b->nks[k] = 1;
+  }
+  if (!b->use_graph) {
+    b->drop_graph();
+    batch_prepare(b);
+    batch_enqueue(b);
+    return;
+  }
+  // a fresh plan may reallocate buffers; recapture when anything moved
+  std::vector<uint64_t> fp = batch_prepare(b);
+  if (!b->exec || fp != b->fingerprint) {
+    b->drop_graph();
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      batch_enqueue(b);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->fingerprint = fp;
+  }
+  HIPCHK(hipGraphLaunch(b->exec, b->stream));
+}
+
+double batch_time(ed_batch* b, int reps) {
+  if (!b->ev[0]) {
+    HIPCHK(hipEventCreate(&b->ev[0]));
+    HIPCHK(hipEventCreate(&b->ev[1]));
+  }
+  batch_run(b);  // warm-up (and capture)
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipEventRecord(b->ev[0], b->stream));
+  for (int r = 0; r < reps; ++r) batch_run(b);
+  HIPCHK(hipEventRecord(b->ev[1], b->stream));
+  HIPCHK(hipEventSynchronize(b->ev[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  return (double)ms / reps;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* ed_version(void) { return "elemental_hip 0.1 (gfx950)"; }
@@ -767,6 +1006,7 @@ int ed_set_stream(ed_context* ctx, void* s) {
 int ed_set_lattice(ed_context* ctx, int Lx, int Ly, int Lz, int Lt, int Ne) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     if (Lx < 1 || Ly < 1 || Lz < 1 || Lt < 1 || Ne < 1) fail(ED_ERR_ARG, "lattice sizes and Ne must be >= 1");
     const long v3 = (long)Lx * Ly * Lz;
     if (v3 > (1L << 26)) fail(ED_ERR_ARG, "spatial volume %ld too large", v3);
@@ -795,6 +1035,7 @@ int ed_set_lattice(ed_context* ctx, int Lx, int Ly, int Lz, int Lt, int Ne) {
 int ed_set_gauge(ed_context* ctx, const void* U, int dtype, int on_device) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     require(ctx->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
     if (!U) fail(ED_ERR_ARG, "U is null");
     if (dtype != ED_C128 && dtype != ED_C64) fail(ED_ERR_ARG, "dtype must be ED_C128 or ED_C64");
@@ -821,6 +1062,7 @@ int ed_set_gauge(ed_context* ctx, const void* U, int dtype, int on_device) {
 int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     require(ctx->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
     if (!V) fail(ED_ERR_ARG, "V is null");
     if (dtype != ED_C128 && dtype != ED_C64) fail(ED_ERR_ARG, "dtype must be ED_C128 or ED_C64");
@@ -865,6 +1107,7 @@ int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device) {
 int ed_set_momenta(ed_context* ctx, const int32_t* n, int P) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     if (P < 1 || !n) fail(ED_ERR_ARG, "momentum list must be non-empty");
     if (P > 4096) fail(ED_ERR_ARG, "at most 4096 momenta");
     ctx->moms.assign(n, n + 3 * P);
@@ -876,6 +1119,7 @@ int ed_set_momenta(ed_context* ctx, const int32_t* n, int P) {
 int ed_set_derivatives(ed_context* ctx, int num_nabla) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     if (num_nabla < 0 || num_nabla > 3) fail(ED_ERR_ARG, "num_nabla must be in [0, 3]");
     ctx->num_nabla = num_nabla;
     ctx->D = level_start(num_nabla + 1);
@@ -891,6 +1135,7 @@ int ed_set_derivatives(ed_context* ctx, int num_nabla) {
 int ed_set_weights(ed_context* ctx, const double* coeff) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     require(ctx->lattice_set, ED_ERR_STATE, "ed_set_lattice must be called first");
     HIPCHK(hipSetDevice(ctx->device));
     if (!coeff) {
@@ -907,6 +1152,7 @@ int ed_set_weights(ed_context* ctx, const double* coeff) {
 int ed_set_option(ed_context* ctx, int option, int64_t value) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     switch (option) {
       case ED_OPT_HERMITIAN:
         ctx->hermitian = value ? 1 : 0;
@@ -1043,6 +1289,7 @@ int ed_profile_reset(ed_context* ctx) {
 int ed_project_su3(ed_context* ctx, int max_iter, int* iterations) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     require(ctx->gauge_set, ED_ERR_STATE, "ed_set_gauge must be called first");
     if (max_iter < 0) fail(ED_ERR_ARG, "max_iter must be >= 0");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1072,6 +1319,7 @@ int ed_project_su3(ed_context* ctx, int max_iter, int* iterations) {
 int ed_stout_smear(ed_context* ctx, int nstep, double rho) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
+    ++ctx->gen;
     require(ctx->gauge_set, ED_ERR_STATE, "ed_set_gauge must be called first");
     if (nstep < 0) fail(ED_ERR_ARG, "nstep must be >= 0");
     if (!std::isfinite(rho)) fail(ED_ERR_ARG, "rho must be finite");
@@ -1102,6 +1350,137 @@ int ed_get_gauge(ed_context* ctx, void* U, int on_device) {
     HIPCHK(hipMemcpyAsync(U, ctx->U64.p, n * sizeof(double2), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ed_batch_create(int device, ed_batch** b) {
+  if (!b) return ED_ERR_ARG;
+  *b = nullptr;
+  ed_batch* x = new (std::nothrow) ed_batch();
+  if (!x) return ED_ERR_ALLOC;
+  int rc = bguard(x, [&] {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(ED_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    x->device = device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+    int cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device));
+    x->ncu = std::max(8, cu);
+  });
+  if (rc != ED_OK) {
+    delete x;
+    return rc;
+  }
+  *b = x;
+  return ED_OK;
+}
+
+int ed_batch_destroy(ed_batch* b) {
+  if (!b) return ED_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->drop_graph();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return ED_OK;
+}
+
+const char* ed_batch_last_error(const ed_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int ed_batch_add(ed_batch* b, ed_context* ctx, int mode, void* out_device) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] {
+    if (!ctx || !out_device) fail(ED_ERR_ARG, "context and output must be non-null");
+    if (mode != ED_MODE_FP64 && mode != ED_MODE_MIXED) fail(ED_ERR_ARG, "unknown mode %d", mode);
+    if (ctx->device != b->device) fail(ED_ERR_ARG, "context is on device %d, batch on %d", ctx->device, b->device);
+    for (auto& it : b->items)
+      if (it.ctx == ctx) fail(ED_ERR_ARG, "a context can appear in a batch only once");
+    b->items.push_back({ctx, mode, (double2*)out_device});
+    b->drop_graph();
+  });
+}
+
+int ed_batch_set_option(ed_batch* b, int option, int64_t value) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] {
+    switch (option) {
+      case ED_BOPT_GRID_PER_CU:
+        if (value < 0 || value > 64) fail(ED_ERR_ARG, "workgroups per CU must be in [0, 64]");
+        b->grid_per_cu = (int)value;
+        break;
+      case ED_BOPT_ORDER:
+        if (value < 0 || value > 1) fail(ED_ERR_ARG, "item order must be 0 or 1");
+        b->order = (int)value;
+        break;
+      case ED_BOPT_GRAPH:
+        b->use_graph = value ? 1 : 0;
+        break;
+      default:
+        fail(ED_ERR_ARG, "unknown batch option %d", option);
+    }
+    b->drop_graph();
+  });
+}
+
+int ed_batch_get_option(ed_batch* b, int option, int64_t* value) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] {
+    if (!value) fail(ED_ERR_ARG, "value is null");
+    switch (option) {
+      case ED_BOPT_GRID_PER_CU: *value = b->grid_per_cu; break;
+      case ED_BOPT_ORDER: *value = b->order; break;
+      case ED_BOPT_GRAPH: *value = b->use_graph; break;
+      default: fail(ED_ERR_ARG, "unknown batch option %d", option);
+    }
+  });
+}
+
+int ed_batch_run(ed_batch* b) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] { batch_run(b); });
+}
+
+int ed_batch_synchronize(ed_batch* b) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] {
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  });
+}
+
+int ed_batch_time(ed_batch* b, int reps, double* ms_per_run) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] {
+    if (reps < 1 || !ms_per_run) fail(ED_ERR_ARG, "reps must be >= 1 and ms_per_run non-null");
+    *ms_per_run = batch_time(b, reps);
+  });
+}
+
+int ed_batch_tune(ed_batch* b, int reps, double* best_ms) {
+  if (!b) return ED_ERR_ARG;
+  return bguard(b, [&] {
+    if (reps < 1) fail(ED_ERR_ARG, "reps must be >= 1");
+    const int grids[] = {1, 2, 4, 0};
+    double best = 1e300;
+    int bg = b->grid_per_cu, bo = b->order;
+    for (int gsel : grids)
+      for (int o = 0; o < 2; ++o) {
+        b->grid_per_cu = gsel;
+        b->order = o;
+        b->drop_graph();
+        const double ms = batch_time(b, reps);
+        if (ms < best) {
+          best = ms;
+          bg = gsel;
+          bo = o;
+        }
+      }
+    b->grid_per_cu = bg;
+    b->order = bo;
+    b->drop_graph();
+    if (best_ms) *best_ms = best;
   });
 }
 

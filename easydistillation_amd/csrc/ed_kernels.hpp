@@ -236,14 +236,10 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
 // while the current chunk is contracted; LDS is double-buffered so one
 // barrier per chunk suffices.
 template <typename T, int NKS>
-__global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* __restrict__ W, const T* __restrict__ phr,
-                                                           const T* __restrict__ phi, double2* __restrict__ out,
-                                                           ContractArgs a) {
+__device__ __forceinline__ void contract_body(const c_t<T>* __restrict__ W, const T* __restrict__ phr,
+                                              const T* __restrict__ phi, double2* __restrict__ out,
+                                              const ContractArgs& a, const int w, c_t<T> (*Qs)[SC][16][16]) {
   typedef typename mfma<T>::acc acc_t;
-  __shared__ c_t<T> Qs[2][SC][16][16];
-
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
-  if (w >= a.nwork) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g4 = lane >> 4;
 
@@ -563,18 +559,12 @@ __device__ __forceinline__ void split16x2(float x0, float x1, uint32_t& hi, uint
 }
 
 template <int NKS>
-__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const float2* __restrict__ W,
-                                                                            const half8* __restrict__ ph16,
-                                                                            double2* __restrict__ out,
-                                                                            ContractArgs a) {
+__device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W, const half8* __restrict__ ph16,
+                                                    double2* __restrict__ out, const ContractArgs& a, const int w,
+                                                    float2 (*Qs)[SC2][16][16], float (*wmax)[NWAVE]) {
   typedef float4v acc_t;
   typedef float v2f __attribute__((ext_vector_type(2)));
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-  __shared__ float2 Qs[2][SC2][16][16];
-  __shared__ float wmax[2][NWAVE];
-
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
-  if (w >= a.nwork) return;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;
 
@@ -855,6 +845,100 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
             make_double2(s * zre * wfe, -s * zim * wfe);
       }
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Launch forms of the contraction.
+//
+// contract_kernel / contract_mixed_f16_kernel: one workgroup per work item
+// (grid = nwork rounded up to 8), items spread over the XCDs in contiguous
+// ranges (xcd_remap).
+//
+// contract_group_kernel / contract_mixed_group_kernel: one persistent launch
+// over several problems (lattices of different size and Ne).  Work items of
+// all problems are numbered problem after problem (work0 = first item of a
+// problem); workgroup b on XCD b % 8 walks the items of its XCD:
+//   order 0: XCD x owns the contiguous range [x n8, (x+1) n8), n8 = ceil(n/8)
+//   order 1: item = 8 k + x (round robin over XCDs)
+// with stride gridDim/8, so any grid that is a multiple of 8 covers all
+// items.  Every workgroup reaches the end of its range and returns.
+// --------------------------------------------------------------------------
+struct GroupProblem {
+  ContractArgs a;
+  const void* W;    // c_t<T> workspace (float2 for the mixed kernel)
+  const void* ph0;  // real phase table (T), or the FP16 split table
+  const void* ph1;  // imaginary phase table (T), unused by the FP16 kernel
+  double2* out;
+  int work0;
+  int pad[3];
+};
+
+__device__ __forceinline__ int group_item(int k, int per, int order) {
+  const int x = blockIdx.x & 7;
+  return order ? 8 * k + x : x * per + k;
+}
+
+__device__ __forceinline__ int group_find(const GroupProblem* __restrict__ probs, int nprob, int item) {
+  int i = 0;
+  while (i + 1 < nprob && probs[i + 1].work0 <= item) ++i;
+  return i;
+}
+
+template <typename T, int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* __restrict__ W, const T* __restrict__ phr,
+                                                                 const T* __restrict__ phi, double2* __restrict__ out,
+                                                                 ContractArgs a) {
+  __shared__ c_t<T> Qs[2][SC][16][16];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= a.nwork) return;
+  contract_body<T, NKS>(W, phr, phi, out, a, w, Qs);
+}
+
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const float2* __restrict__ W,
+                                                                            const half8* __restrict__ ph16,
+                                                                            double2* __restrict__ out,
+                                                                            ContractArgs a) {
+  __shared__ float2 Qs[2][SC2][16][16];
+  __shared__ float wmax[2][NWAVE];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= a.nwork) return;
+  contract_mixed_body<NKS>(W, ph16, out, a, w, Qs, wmax);
+}
+
+template <typename T, int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_group_kernel(const GroupProblem* __restrict__ probs, int nprob,
+                                                                       int total, int order) {
+  __shared__ c_t<T> Qs[2][SC][16][16];
+  const int per = (total + 7) >> 3;
+  const int nj = gridDim.x >> 3;
+  for (int k = blockIdx.x >> 3; k < per; k += nj) {
+    const int item = group_item(k, per, order);
+    if (item >= total) break;
+    const int i = group_find(probs, nprob, item);
+    const ContractArgs a = probs[i].a;
+    __syncthreads();  // the previous item's LDS reads are done
+    contract_body<T, NKS>((const c_t<T>*)probs[i].W, (const T*)probs[i].ph0, (const T*)probs[i].ph1, probs[i].out, a,
+                          item - probs[i].work0, Qs);
+  }
+}
+
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(const GroupProblem* __restrict__ probs,
+                                                                             int nprob, int total, int order) {
+  __shared__ float2 Qs[2][SC2][16][16];
+  __shared__ float wmax[2][NWAVE];
+  const int per = (total + 7) >> 3;
+  const int nj = gridDim.x >> 3;
+  for (int k = blockIdx.x >> 3; k < per; k += nj) {
+    const int item = group_item(k, per, order);
+    if (item >= total) break;
+    const int i = group_find(probs, nprob, item);
+    const ContractArgs a = probs[i].a;
+    __syncthreads();
+    contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                             item - probs[i].work0, Qs, wmax);
   }
 }
 

@@ -203,6 +203,42 @@ int ed_project_su3(ed_context* ctx, int max_iter, int* iterations);
 int ed_stout_smear(ed_context* ctx, int nstep, double rho);
 int ed_get_gauge(ed_context* ctx, void* U, int on_device);
 
+/* Batched form (no counterpart in the reference, whose callers loop over
+ * configurations and timeslices one calc(t) at a time): several contexts --
+ * lattices of different size, Ne, momenta and derivative depth, each with its
+ * inputs set -- computed together.  One run enqueues, for every context, the
+ * staging and hop kernels over all its timeslices and then one persistent,
+ * grouped contraction launch over the work items of every context; with
+ * ED_BOPT_GRAPH = 1 (default) this sequence is captured once into a hipGraph
+ * and each run is a single graph launch (recaptured automatically when a
+ * context's inputs, plan or buffers change).  Each context holds the
+ * workspace of all its timeslices at once.
+ *   ed_batch_add: out_device is a device array [Lt][D][P][Ne][Ne] complex128
+ *                 receiving every timeslice of ctx in mode (FP64 or MIXED).
+ *   ed_batch_run: enqueue on the batch's own stream; ed_batch_synchronize waits.
+ *   ed_batch_time: mean duration (HIP events) of reps runs after one warm-up.
+ *   ed_batch_tune: autotuner -- times every launch shape (persistent grid of
+ *                 1, 2 or 4 workgroups per CU, or one workgroup per work item)
+ *                 and item order (contiguous ranges per XCD, or round robin
+ *                 over XCDs) with reps runs each, keeps the fastest and
+ *                 returns its time. */
+typedef struct ed_batch ed_batch;
+enum ed_batch_option {
+  ED_BOPT_GRID_PER_CU = 1, /* persistent workgroups per CU (0: one per work item; default 2) */
+  ED_BOPT_ORDER = 2,       /* 0: contiguous item ranges per XCD (default), 1: round robin     */
+  ED_BOPT_GRAPH = 3        /* 1: capture and replay a hipGraph (default), 0: direct launches  */
+};
+int ed_batch_create(int device, ed_batch** b);
+int ed_batch_destroy(ed_batch* b);
+const char* ed_batch_last_error(const ed_batch* b);
+int ed_batch_add(ed_batch* b, ed_context* ctx, int mode, void* out_device);
+int ed_batch_set_option(ed_batch* b, int option, int64_t value);
+int ed_batch_get_option(ed_batch* b, int option, int64_t* value);
+int ed_batch_run(ed_batch* b);
+int ed_batch_synchronize(ed_batch* b);
+int ed_batch_time(ed_batch* b, int reps, double* ms_per_run);
+int ed_batch_tune(ed_batch* b, int reps, double* best_ms);
+
 /* Library version string (static storage). */
 const char* ed_version(void);
 
