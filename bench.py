@@ -37,6 +37,12 @@ WORKLOADS = {
     "B": ([16, 16, 16, 128], 100, 2, 4, "smooth"),
     "small": ([16, 16, 16, 8], 64, 2, 4, "haar"),
 }
+# batched uneven form: three lattices in one graph launch, num_nabla 1 (4
+# derivative combinations), 19 momenta (|n|^2 <= 2), smooth links eps = 0.3
+BATCHED = {
+    "D": [([16, 16, 16, 32], 64), ([24, 24, 24, 48], 128), ([32, 32, 32, 64], 256)],
+    "D-small": [([8, 8, 8, 4], 16), ([12, 12, 12, 6], 32), ([16, 16, 16, 8], 64)],
+}
 
 
 def log(msg):
@@ -125,12 +131,135 @@ def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
     }
 
 
+def bench_batched(args):
+    """Batched uneven form: every timeslice of three lattices per graph launch (mixed form).
+
+    One step = one replay of the batch's hipGraph (staging and hops of each
+    lattice, then one persistent grouped contraction).  value = sum over
+    lattices of Lt P D Ne^2 / seconds per step.  Each lattice is also timed
+    alone (its own tuned batch) for the per-lattice figures, and every batch
+    output is compared with the FP64 form of the same lattice.
+    """
+    import torch
+
+    import easydistillation_amd as ed
+    from easydistillation_amd import _native
+    from easydistillation_amd import synthetic as S
+    from easydistillation_amd.batch import ElementalBatch
+
+    dev = torch.device("cuda:0")
+    moms = S.momenta_by_norm(2)
+    num_nabla = 1
+    D = ed.num_derivative(num_nabla)
+    P = len(moms)
+    gens = []
+    for i, (latt, Ne) in enumerate(BATCHED[args.workload]):
+        t0 = time.time()
+        U = S.gauge_torch(latt, "smooth", 0.3, seed=21 + i, device=dev)
+        torch.cuda.synchronize()
+        log(f"lattice {latt}: links in {time.time() - t0:.1f} s")
+        V = S.eigvecs_torch(latt, Ne, seed=31 + i, device=dev, log=log)
+        g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode="mixed")
+        g.load_arrays(U, V)
+        del U, V
+        gens.append(g)
+        log(f"lattice {latt} Ne={Ne}: inputs in {time.time() - t0:.1f} s")
+    torch.cuda.synchronize()
+
+    batch = ElementalBatch(gens)
+    t0 = time.time()
+    tuned_ms = batch.tune(1)
+    shape = {"grid_per_cu": batch.option(_native.ED_BOPT_GRID_PER_CU), "order": batch.option(_native.ED_BOPT_ORDER)}
+    log(f"autotuned in {time.time() - t0:.1f} s: {tuned_ms:.1f} ms per run, {shape}")
+    for i in range(args.warmup):
+        batch.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batch.batch.run()
+        batch.batch.synchronize()
+        log(f"batched step {i} done at {time.perf_counter() - t0:.2f} s")
+    dt = (time.perf_counter() - t0) / args.steps
+    elems = batch.elementals()
+    total = sum(elems)
+
+    per = []
+    f32 = f16 = 0
+    for g, e in zip(gens, elems):
+        nt = g.latt_size[3]
+        f32_t = g.context.query(_native.ED_Q_MIXED_F32_FLOPS_PER_T)
+        f16_t = g.context.query(_native.ED_Q_MIXED_F16_FLOPS_PER_T)
+        f32 += f32_t * nt
+        f16 += f16_t * nt
+        single = ElementalBatch([g])
+        single.tune(1)
+        ms = single.time(max(1, args.steps))
+        per.append(
+            {
+                "lattice": "x".join(str(v) for v in g.latt_size),
+                "Nev": g.Ne,
+                "elementals_per_run": e,
+                "ms_alone": ms,
+                "elementals_per_s_alone": e / (ms * 1e-3),
+            }
+        )
+        del single
+        log(f"alone {per[-1]}")
+    equiv = f32 + f16 * PEAK["mixed"] / PEAK_FP16
+
+    # error of each batch output against the FP64 form of the same lattice
+    outs = batch.run()
+    errs = []
+    for g, o in zip(gens, outs):
+        ref = g.calc_all(mode="fp64", out=torch.empty_like(o))
+        errs.append(blockwise_error(o, ref)[0])
+        del ref
+    log(f"mixed batch vs fp64 blockwise error per lattice: {errs}")
+
+    line = {
+        "metric": "elementals/s (t x p x d x Nev^2), batched uneven lattices, mixed form, 1 GPU",
+        "value": total / dt,
+        "unit": "elementals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded smooth SU(3) links eps=0.3, orthonormal complex64 eigenvectors)",
+        "config": {
+            "workload": " + ".join(f"{p['lattice']}/Nev{p['Nev']}" for p in per),
+            "momenta": P,
+            "derivatives": D,
+            "form": "mixed",
+            "launch": "one hipGraph replay per step",
+            "tuned_launch_shape": shape,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": equiv / dt / 1e12,
+            "peak": PEAK["mixed"],
+            "unit": "TFLOP/s",
+            "frac": equiv / dt / 1e12 / PEAK["mixed"],
+            "traffic": None,
+            "kernel": "whole graph (stencils + contract_mixed_group_kernel)",
+            "flops_basis": "executed FP32 MFMA flops plus FP16 MFMA flops at the 157.3:2500 peak ratio, over the whole step",
+        },
+        "per_lattice": per,
+        "mixed_vs_fp64_error_max": max(errs),
+        "mixed_vs_fp64_error_per_lattice": errs,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS) + sorted(BATCHED))
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -151,6 +280,10 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+
+    if args.workload in BATCHED:
+        bench_batched(args)
+        return
 
     import torch
 

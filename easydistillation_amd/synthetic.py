@@ -130,7 +130,7 @@ def gauge_torch(latt_size, kind="haar", eps=0.3, seed=0, device="cuda", chunk=1 
     haar:   Gram-Schmidt on the first two columns of a complex Gaussian 3x3
             matrix, third column conj(u1 x u2), so det = 1 (Haar on SU(3));
     smooth: exp(i eps H) with H the traceless Hermitian part of a complex
-            Gaussian, through the Hermitian eigendecomposition;
+            Gaussian (batched matrix exponential);
     unit:   identity links.
     """
     import torch
@@ -164,8 +164,7 @@ def gauge_torch(latt_size, kind="haar", eps=0.3, seed=0, device="cuda", chunk=1 
             H = 0.5 * (Z + Z.conj().transpose(-1, -2))
             tr = torch.diagonal(H, dim1=-2, dim2=-1).sum(-1).real / 3.0
             H = H - tr[:, None, None] * eye
-            w, Qh = torch.linalg.eigh(H)
-            Q = (Qh * torch.exp(1j * eps * w).unsqueeze(-2)) @ Qh.conj().transpose(-1, -2)
+            Q = torch.linalg.matrix_exp((1j * eps) * H)
         elif kind == "unit":
             Q = eye.expand(m, 3, 3)
         else:
