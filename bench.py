@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS) + sorted(BATCHED))
     ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--mixed-ws", type=int, default=1, choices=(0, 1),
+                    help="mixed form's FP16 stage-2 kernel: 1 warp-specialized (default), 0 uniform waves")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument(
@@ -322,6 +324,7 @@ def main():
     outs = {}
     for form in forms:
         g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode=form)
+        g.set_option(_native.ED_OPT_MIXED_WS, args.mixed_ws)
         g.load_arrays(U, V)
         out = torch.empty((Lt, D, P, Ne, Ne), dtype=torch.complex128, device=dev)
         for i in range(args.warmup):

@@ -137,6 +137,7 @@ struct ed_context {
   DevBuf<float> ph32;
   DevBuf<ed::half8> ph16;  // [npb][V3P/32][4][16][32] fp16 hi/lo of 2^14 Phi
   int mixed_stage2 = -1;   // 1: FP16 three-term split, 0: FP32 MFMA, -1: FP16 from V3 >= 4096 on
+  int mixed_ws = 1;        // FP16 stage 2: 1 warp-specialized kernel (up to 3 stage-1 k-steps), 0 uniform waves
   int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
   uint64_t gen = 0;        // bumped whenever buffers or plan a captured launch refers to change
 
@@ -574,7 +575,14 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     if (sizeof(T) == 4 && use_f16_stage2(c)) {
       const float2* Wf = (const float2*)(const void*)W;
       c->launch(ED_KERNEL_CONTRACT, [&] {
-        if (c->nks_max <= 1)
+        if (c->mixed_ws && c->nks_max <= 3) {
+          if (c->nks_max <= 1)
+            ed::contract_mixed_ws_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+          else if (c->nks_max <= 2)
+            ed::contract_mixed_ws_kernel<2><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+          else
+            ed::contract_mixed_ws_kernel<3><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+        } else if (c->nks_max <= 1)
           ed::contract_mixed_f16_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
         else if (c->nks_max <= 2)
           ed::contract_mixed_f16_kernel<2><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
@@ -1168,6 +1176,9 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
         break;
       case ED_OPT_PROFILE:
         ctx->profile = value ? 1 : 0;
+        break;
+      case ED_OPT_MIXED_WS:
+        ctx->mixed_ws = value ? 1 : 0;
         break;
       case ED_OPT_EXACT_V:
         ctx->exact_v = value ? 1 : 0;

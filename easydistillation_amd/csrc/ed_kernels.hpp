@@ -643,9 +643,10 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
     for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
   }
 
-  // stage 2 of one chunk for N-block b; FULL: no padded site in the chunk
-  auto stage2 = [&](auto FULL, int b, int c0, float sq, int ex) {
-    constexpr bool full = decltype(FULL)::value;
+  // stage 2 of one chunk for N-block b.  Padded sites hold Q = 0 and Phi = 0:
+  // their (0 - c) sq terms meet a zero phase and only enter the p = 0 sum,
+  // which the epilogue corrects with V3P in place of V3.
+  auto stage2 = [&](int b, int c0, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
     float zsr = 0.f, zsi = 0.f;
@@ -659,9 +660,6 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
         // scaled and mean-subtracted value s = (q - c) sq; padded sites give 0
         sr[h] = fmaf(q.x, sq, csr);
         si[h] = fmaf(q.y, sq, csi);
-        if constexpr (!full) {
-          if ((c0 + xl) >= g.V3) sr[h] = si[h] = 0.f;
-        }
         zsr += sr[h];
         zsi += si[h];
       }
@@ -788,13 +786,8 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
     if (chunk + 1 < nchunk) stage1(chunk + 1, buf ^ 1);
 
     // ---- stage 2 (FP16 MFMA, three-term split) of chunk ----
-    if (c0 + SC2 <= g.V3) {
 #pragma unroll
-      for (int b = 0; b < NBW; ++b) stage2(std::true_type{}, b, c0, sq, ex);
-    } else {
-#pragma unroll
-      for (int b = 0; b < NBW; ++b) stage2(std::false_type{}, b, c0, sq, ex);
-    }
+    for (int b = 0; b < NBW; ++b) stage2(b, c0, sq, ex);
     if (chunk + 1 < nchunk) load_phase(chunk + 1);
     __syncthreads();
   }
@@ -835,8 +828,368 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
     zim += __shfl_xor(zim, 16);
     zre += __shfl_xor(zre, 32);
     zim += __shfl_xor(zim, 32);
-    zre += (double)cr[b] * g.V3;
-    zim += (double)ci[b] * g.V3;
+    zre += (double)cr[b] * g.V3P;  // the V3P - V3 padded sites entered as (0 - c)
+    zim += (double)ci[b] * g.V3P;
+    if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
+      out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
+            make_double2(s * zre * wfe, -s * zim * wfe);
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Mixed form, warp-specialized.  Waves 0..3 (producers) run stage 1 -- the
+// FP32 MFMA split sum, 8 sites each per 32-site chunk -- and publish Q and
+// its tile maximum in LDS; waves 4..7 (consumers) run stage 2 -- scaling,
+// FP16 hi/lo split and FP16 MFMA phase contraction, 4 eigenvector rows each.
+// Wave w and wave w + 4 share a SIMD, so one wave's conversion work overlaps
+// the other's matrix instructions.  The roles keep separate loops with the
+// same number of barriers (1 + nchunk), so neither role's registers are live
+// in the other.  Numerics, LDS layout and epilogue are those of
+// contract_mixed_body.
+// --------------------------------------------------------------------------
+constexpr int NPROD = 4;                   // producer waves
+// LDS row-major Q tile per site, padded by 16 bytes: the consumer's four lane
+// groups read sites 8 apart, 8 x 2064 bytes = 128 (mod 256), so they fall in
+// alternate halves of the 64 banks
+constexpr int QSITE = 16 * 16 + 2;
+constexpr int SPP = SC2 / NPROD;           // stage-1 sites per producer wave and chunk
+constexpr int NBC = 16 / (NWAVE - NPROD);  // stage-2 rows per consumer wave
+
+// integer maximum over the wave of a non-negative float (bit patterns of
+// non-negative floats order like the floats); every lane receives it
+template <int CTRL, int RMASK>
+__device__ __forceinline__ int dpp_imax_step(int x) {
+  const int y = __builtin_amdgcn_update_dpp(x, x, CTRL, RMASK, 0xF, false);
+  return max(x, y);
+}
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  int x = __float_as_int(v);
+  x = dpp_imax_step<0x111, 0xF>(x);
+  x = dpp_imax_step<0x112, 0xF>(x);
+  x = dpp_imax_step<0x114, 0xF>(x);
+  x = dpp_imax_step<0x118, 0xF>(x);
+  x = dpp_imax_step<0x142, 0xA>(x);
+  x = dpp_imax_step<0x143, 0xC>(x);
+  return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
+// Producer loop of contract_mixed_ws_body for exactly NK stage-1 k-steps:
+// no branch inside the chunk loop, so the loads of the next chunk (issued as
+// soon as this chunk's copy of each operand is consumed; the last chunk
+// re-reads itself) stay in flight across the MFMAs and the barrier.
+template <int NK>
+__device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, const ContractArgs& a, const DPlan& dp,
+                                                  int tb, int E0, int F0, float2 (*Qs)[SC2][QSITE],
+                                                  float (*wmax)[NWAVE]) {
+  typedef float4v acc_t;
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l16 = lane & 15, g4 = lane >> 4;
+  const Geom g = a.g;
+  const int nchunk = g.V3P / SC2;
+  const acc_t zero = {0.f, 0.f, 0.f, 0.f};
+  const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;     // bytes per site
+  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 (checked on the host)
+  const float2* Wt = W + (size_t)tb * (slice_b / 8);
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
+  const int J = 3 * dp.nsplit;
+  int offL[NK], offR[NK];
+  float cf[NK];
+#pragma unroll
+  for (int ks = 0; ks < NK; ++ks) {
+    const int j = 4 * ks + g4;
+    if (j < J) {
+      const SplitTerm st = a.splits[dp.first + j / 3];
+      const int c = j % 3;
+      offL[ks] = (int)(widx(g, st.nL, 0, c, E0 + l16) * 8);
+      offR[ks] = (int)(widx(g, st.nR, 0, c, F0 + l16) * 8);
+      cf[ks] = st.coeff;
+    } else {
+      offL[ks] = offR[ks] = 0;
+      cf[ks] = 0.f;
+    }
+  }
+  v2f lv[NK][SPP], rv[NK][SPP];
+  auto load = [&](int ks, bool left, int c0) {
+    const int xo = (c0 + SPP * wave) * (int)site_b;  // uniform
+#pragma unroll
+    for (int i = 0; i < SPP; ++i) {
+      if (left)
+        lv[ks][i] = __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(wr, offL[ks], xo + i * (int)site_b, 0));
+      else
+        rv[ks][i] = __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(wr, offR[ks], xo + i * (int)site_b, 0));
+    }
+  };
+  // zero-momentum sums of this wave's sites, rows 4 g4 + r, column l16, of
+  // Q - z0 with z0 this lane's mean over its sites of chunk 0 (added back in
+  // FP64 at the end), so the FP32 partial sums carry only fluctuations
+  double zr[4] = {0, 0, 0, 0}, zi[4] = {0, 0, 0, 0};
+  float z0r[4] = {0.f, 0.f, 0.f, 0.f}, z0i[4] = {0.f, 0.f, 0.f, 0.f};
+
+  auto stage1 = [&](int chunk, int buf, bool first) {
+    const int c0n = min(chunk + 1, nchunk - 1) * SC2;  // the last chunk re-reads itself
+    acc_t qr[SPP], qi[SPP];
+    float lr[SPP], li[SPP];
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) {
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) {
+        lr[i] = cf[ks] * lv[ks][i].x;
+        li[i] = cf[ks] * lv[ks][i].y;
+      }
+      load(ks, true, c0n);
+      // conj(L) R = (lr rr + li ri) + i (lr ri - li rr)
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) qr[i] = mfma<float>::mac(lr[i], rv[ks][i].x, ks ? qr[i] : zero);
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) qi[i] = mfma<float>::mac(lr[i], rv[ks][i].y, ks ? qi[i] : zero);
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) qr[i] = mfma<float>::mac(li[i], rv[ks][i].y, qr[i]);
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) qi[i] = mfma<float>::mac(-li[i], rv[ks][i].x, qi[i]);
+      load(ks, false, c0n);
+    }
+    // Q to LDS, the tile maximum and the zero-momentum sums (padded sites
+    // hold Q = 0 exactly)
+    if (first) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float ar = 0.f, ai = 0.f;
+#pragma unroll
+        for (int i = 0; i < SPP; ++i) {
+          ar += qr[i][r];
+          ai += qi[i][r];
+        }
+        z0r[r] = ar * (1.f / SPP);
+        z0i[r] = ai * (1.f / SPP);
+      }
+    }
+    float m = 0.f;
+    float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < SPP; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Qs[buf][SPP * wave + i][mfma<float>::row(g4, r) * 16 + l16] = make_float2(qr[i][r], qi[i][r]);
+        sr[r] += qr[i][r] - z0r[r];
+        si[r] += qi[i][r] - z0i[r];
+      }
+      m = fmaxf(fmaxf(m, fabsf(qr[i][0])), fabsf(qi[i][0]));
+      m = fmaxf(fmaxf(m, fabsf(qr[i][1])), fabsf(qi[i][1]));
+      m = fmaxf(fmaxf(m, fabsf(qr[i][2])), fabsf(qi[i][2]));
+      m = fmaxf(fmaxf(m, fabsf(qr[i][3])), fabsf(qi[i][3]));
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      zr[r] += (double)sr[r];
+      zi[r] += (double)si[r];
+    }
+    m = wave_max_nonneg(m);
+    if (lane == 0) wmax[buf][wave] = m;
+  };
+
+  // issue priority over the consumer wave of the same SIMD: the matrix pipe
+  // is kept fed by the split-sum MFMAs, the consumer fills VALU slots
+  __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+  for (int ks = 0; ks < NK; ++ks) {
+    load(ks, true, 0);
+    load(ks, false, 0);
+  }
+  stage1(0, 0, true);
+  __syncthreads();
+  for (int chunk = 0; chunk + 1 < nchunk; ++chunk) {
+    stage1(chunk + 1, (chunk + 1) & 1, false);
+    __syncthreads();
+  }
+  __syncthreads();  // matches the consumers' last chunk
+  // every consumer has read its last chunk: LDS buffer 0 takes the four
+  // producers' zero-momentum partial sums, rows x columns, as doubles
+  double2* zs = (double2*)(void*)&Qs[0][0][0];
+  const double nsum = (double)SPP * nchunk;  // sites this wave summed, padded ones (Q = 0) included
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    zs[(wave * 16 + mfma<float>::row(g4, r)) * 16 + l16] =
+        make_double2(fma((double)z0r[r], nsum, zr[r]), fma((double)z0i[r], nsum, zi[r]));
+  __syncthreads();
+}
+
+template <int NKS>
+__device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict__ W, const half8* __restrict__ ph16,
+                                                       double2* __restrict__ out, const ContractArgs& a, const int w,
+                                                       float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE]) {
+  typedef float4v acc_t;
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l16 = lane & 15, g4 = lane >> 4;
+
+  int rest = w;
+  const int tf = rest % a.ntf; rest /= a.ntf;
+  const int te = rest % a.nte; rest /= a.nte;
+  const int pb = rest % a.npb; rest /= a.npb;
+  const int di = rest % a.nd;
+  const int tb = rest / a.nd;
+  const DPlan dp = a.dplan[di];
+  const int E0 = te * 16, F0 = tf * 16;
+  const Geom g = a.g;
+  const int nchunk = g.V3P / SC2;
+  const acc_t zero = {0.f, 0.f, 0.f, 0.f};
+
+  if (wave < NPROD) {
+    // producer: one branch-free instance per stage-1 k-step count
+    const int nks = (3 * dp.nsplit + 3) >> 2;
+    if (NKS >= 3 && nks == 3) mixed_ws_producer<(NKS >= 3 ? 3 : 1)>(W, a, dp, tb, E0, F0, Qs, wmax);
+    else if (NKS >= 2 && nks == 2) mixed_ws_producer<(NKS >= 2 ? 2 : 1)>(W, a, dp, tb, E0, F0, Qs, wmax);
+    else mixed_ws_producer<1>(W, a, dp, tb, E0, F0, Qs, wmax);
+    return;
+  }
+
+  // ================================ consumer ================================
+  const int cw = wave - NPROD;
+  const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4;
+  const __amdgpu_buffer_rsrc_t pr_ = __builtin_amdgcn_make_buffer_rsrc((void*)PH, (short)0, nchunk * 4 * 16 * 4 * 16, 0x00020000);
+  const int phoff = (l16 * 4 + g4) * 16;
+  half8 prh, prl, pih, pil;
+  auto load_phase = [&](int chunk) {
+    const int co = chunk * 4 * 16 * 4 * 16;  // uniform
+    prh = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 0 * 1024, 0));
+    prl = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 1 * 1024, 0));
+    pih = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 2 * 1024, 0));
+    pil = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 3 * 1024, 0));
+  };
+  double D1[NBC][4], D2[NBC][4], D3[NBC][4], D4[NBC][4];
+  float cr[NBC], ci[NBC];
+#pragma unroll
+  for (int b = 0; b < NBC; ++b) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
+  }
+  // stage 2 of one chunk for row b (padded sites hold Q = 0 and Phi = 0, so
+  // their (0 - c) sq terms meet a zero phase).  qb points at this lane's
+  // first site (8 g4) of the chunk, row 4 cw, column l16: the site (+QSITE
+  // float2) and row (+16) offsets are immediates.
+  auto stage2 = [&](int b, const float2* qb, float sq, int ex) {
+    const float csr = -cr[b] * sq, csi = -ci[b] * sq;
+    uint32_t qrh[4], qrl[4], qih[4], qil[4];
+#pragma unroll
+    for (int v2 = 0; v2 < 4; ++v2) {
+      float sr[2], si[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float2 q = qb[(2 * v2 + h) * QSITE + b * 16];
+        sr[h] = fmaf(q.x, sq, csr);
+        si[h] = fmaf(q.y, sq, csi);
+      }
+      split16x2(sr[0], sr[1], qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], qih[v2], qil[v2]);
+    }
+    const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
+    const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
+    const half8 Qih = __builtin_bit_cast(half8, u4{qih[0], qih[1], qih[2], qih[3]});
+    const half8 Qil = __builtin_bit_cast(half8, u4{qil[0], qil[1], qil[2], qil[3]});
+    acc_t A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qrh, zero, 0, 0, 0);
+    acc_t A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qih, zero, 0, 0, 0);
+    acc_t A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qrh, zero, 0, 0, 0);
+    acc_t A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qih, zero, 0, 0, 0);
+    A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qrl, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qil, A2, 0, 0, 0);
+    A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qrl, A3, 0, 0, 0);
+    A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qil, A4, 0, 0, 0);
+    A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, Qrh, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, Qih, A2, 0, 0, 0);
+    A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, Qrh, A3, 0, 0, 0);
+    A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, Qih, A4, 0, 0, 0);
+    const double inv = ldexp(1.0, ex - 28);  // 1 / (2^14 sq)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      D1[b][r] = fma((double)A1[r], inv, D1[b][r]);
+      D2[b][r] = fma((double)A2[r], inv, D2[b][r]);
+      D3[b][r] = fma((double)A3[r], inv, D3[b][r]);
+      D4[b][r] = fma((double)A4[r], inv, D4[b][r]);
+    }
+  };
+
+  load_phase(0);
+  __syncthreads();
+  {
+    const int n0 = min(SC2, g.V3);
+#pragma unroll
+    for (int b = 0; b < NBC; ++b) {
+      float sr = 0, si = 0;
+      for (int xl = 0; xl < n0; ++xl) {
+        const float2 q = Qs[0][xl][(NBC * cw + b) * 16 + l16];
+        sr += q.x;
+        si += q.y;
+      }
+      cr[b] = sr / (float)n0;
+      ci[b] = si / (float)n0;
+    }
+  }
+  for (int chunk = 0; chunk < nchunk; ++chunk) {
+    const int c0 = chunk * SC2, buf = chunk & 1;
+    float tmax = 0.f;
+#pragma unroll
+    for (int v = 0; v < NPROD; ++v) tmax = fmaxf(tmax, wmax[buf][v]);
+    int ex = 0;
+    if (tmax > 0.f) (void)frexpf(2.f * tmax, &ex);
+    const float sq = ldexpf(1.f, 14 - ex);
+    // one row at a time: no interleaving across rows keeps a single row's LDS
+    // operands, split values and MFMA results live
+    const float2* qb = &Qs[buf][8 * g4][NBC * cw * 16 + l16];
+#pragma unroll
+    for (int b = 0; b < NBC; ++b) {
+      stage2(b, qb, sq, ex);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (chunk + 1 < nchunk) load_phase(chunk + 1);
+    __syncthreads();
+  }
+  __syncthreads();  // the producers' zero-momentum partial sums are in LDS
+  const double2* zs = (const double2*)(const void*)&Qs[0][0][0];
+
+  // ---- epilogue (as contract_mixed_body, rows NBC cw + b) ----
+  const int Ne = g.Ne, P = a.P;
+  const size_t plane = (size_t)Ne * Ne;
+  const size_t tbase = (size_t)(a.t_out0 + tb) * a.D * P;
+  const int f = F0 + l16;
+#pragma unroll
+  for (int b = 0; b < NBC; ++b) {
+    const int e = E0 + NBC * cw + b;
+    const bool ok = (e < Ne) && (f < Ne);
+    const double wef = (ok && a.wgt) ? a.wgt[(size_t)e * Ne + f] : 1.0;
+    const double wfe = (ok && a.wgt) ? a.wgt[(size_t)f * Ne + e] : 1.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int bi = pb * 16 + mfma<float>::row(g4, r);
+      if (!ok || bi >= a.nbase) continue;
+      const int ip = a.plus_idx[bi], im = a.minus_idx[bi];
+      const double S = a.psum[bi], mr = (double)cr[b] * S, mi = (double)ci[b] * S;
+      const double epr = D1[b][r] - D2[b][r] + mr, epi = D3[b][r] + D4[b][r] + mi;
+      const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
+      if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
+      if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        if (im >= 0)
+          out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
+              make_double2(s * epr * wfe, -s * epi * wfe);
+        if (ip >= 0)
+          out[(tbase + (size_t)dp.partner * P + ip) * plane + (size_t)f * Ne + e] =
+              make_double2(s * emr * wfe, -s * emi * wfe);
+      }
+    }
+    // zero momentum: the producers' sums of Q over their sites (no mean
+    // subtraction), each lane g4 takes a quarter of the terms
+    const int row = NBC * cw + b;
+    const double2 z0 = zs[(0 * 16 + row) * 16 + l16], z1 = zs[(1 * 16 + row) * 16 + l16];
+    const double2 z2 = zs[(2 * 16 + row) * 16 + l16], z3 = zs[(3 * 16 + row) * 16 + l16];
+    const double zre = (z0.x + z1.x) + (z2.x + z3.x), zim = (z0.y + z1.y) + (z2.y + z3.y);
     if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
       out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
       if (dp.partner >= 0) {
@@ -907,6 +1260,18 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
   contract_mixed_body<NKS>(W, ph16, out, a, w, Qs, wmax);
 }
 
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_ws_kernel(const float2* __restrict__ W,
+                                                                           const half8* __restrict__ ph16,
+                                                                           double2* __restrict__ out,
+                                                                           ContractArgs a) {
+  __shared__ float2 Qs[2][SC2][QSITE];
+  __shared__ float wmax[2][NWAVE];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= a.nwork) return;
+  contract_mixed_ws_body<NKS>(W, ph16, out, a, w, Qs, wmax);
+}
+
 template <typename T, int NKS>
 __global__ __launch_bounds__(64 * NWAVE, 1) void contract_group_kernel(const GroupProblem* __restrict__ probs, int nprob,
                                                                        int total, int order) {
@@ -927,7 +1292,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_group_kernel(const Gro
 template <int NKS>
 __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(const GroupProblem* __restrict__ probs,
                                                                              int nprob, int total, int order) {
-  __shared__ float2 Qs[2][SC2][16][16];
+  __shared__ float2 Qs[2][SC2][QSITE];
   __shared__ float wmax[2][NWAVE];
   const int per = (total + 7) >> 3;
   const int nj = gridDim.x >> 3;
@@ -937,8 +1302,12 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(con
     const int i = group_find(probs, nprob, item);
     const ContractArgs a = probs[i].a;
     __syncthreads();
-    contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
-                             item - probs[i].work0, Qs, wmax);
+    if constexpr (NKS <= 3)
+      contract_mixed_ws_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                                  item - probs[i].work0, Qs, wmax);
+    else
+      contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                               item - probs[i].work0, (float2(*)[SC2][16][16])(void*)Qs, wmax);
   }
 }
 

@@ -118,14 +118,19 @@ int ed_synchronize(ed_context* ctx);
  *   ED_OPT_EXACT_V        1: the next ed_set_eigvecs also keeps V in complex128 and the FP64
  *                         form and ed_backward use it unrounded (for derivative checks by
  *                         finite differences); 0 (default): V is rounded to complex64 as
- *                         in the reference. */
+ *                         in the reference.
+ *   ED_OPT_MIXED_WS       FP16 stage 2 kernel: 1 (default) warp-specialized (4 waves run the FP32
+ *                         split sum, 4 the FP16 phase contraction, overlapping on each SIMD),
+ *                         0 every wave runs both stages; derivatives with more than 12
+ *                         (split, colour) terms always use the latter. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
   ED_OPT_WS_MB = 3,
   ED_OPT_PROFILE = 4,
   ED_OPT_MIXED_STAGE2 = 5,
-  ED_OPT_EXACT_V = 6
+  ED_OPT_EXACT_V = 6,
+  ED_OPT_MIXED_WS = 7
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 

@@ -234,3 +234,44 @@ def test_mixed_stage2_variants(stage2):
     err = blockwise(E, O.elemental(V, U, latt, 2, moms))
     say(f"  err={err:.3e}")
     assert err <= MIXED_TOL
+
+
+WS_CASES = [
+    # latt, Ne, num_nabla, momenta: padded sites (V3 % 32 != 0), Ne % 16 != 0,
+    # two momentum blocks, every split count up to 12 (split, colour) terms
+    ([16, 16, 16, 2], 24, 2, S.momenta_by_norm(4)),
+    ([5, 7, 9, 2], 37, 2, S.momenta_by_norm(5)[:57]),
+    ([6, 6, 6, 3], 16, 1, S.unit_momenta()),
+    ([9, 4, 3, 1], 9, 0, [(1, 0, 0), (0, 0, 0), (-1, 0, 0)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(WS_CASES)))
+def test_mixed_warp_specialized_matches_uniform(case):
+    """The warp-specialized FP16 stage-2 kernel runs the uniform kernel's arithmetic
+    in the same order for every p != 0: those outputs must agree bit for bit.  The
+    p = 0 sums are formed in another order (per-site sums in the stage-1 waves)
+    and must agree to FP32 rounding of the fluctuations (1e-7)."""
+    latt, Ne, nn, moms = WS_CASES[case]
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=90 + case)
+    say(f"ws vs uniform: latt={latt} Ne={Ne} nabla={nn} P={len(moms)}")
+    outs = []
+    for ws in (1, 0):
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
+        g.set_option(_native.ED_OPT_MIXED_STAGE2, 1)
+        g.set_option(_native.ED_OPT_MIXED_WS, ws)
+        g.load_arrays(U, V)
+        outs.append(g.calc_all())
+    nz = [i for i, m in enumerate(moms) if tuple(m) != (0, 0, 0)]
+    diff = np.abs(outs[0][:, :, nz] - outs[1][:, :, nz]).max()
+    z = [i for i, m in enumerate(moms) if tuple(m) == (0, 0, 0)]
+    dz = blockwise(outs[0][:, :, z], outs[1][:, :, z]) if z else 0.0
+    say(f"  max |ws - uniform| over p != 0: {diff:.1e}; p = 0 blockwise {dz:.1e}")
+    if latt[0] * latt[1] * latt[2] >= 4096:
+        ref = O.elemental(V, U, latt, nn, moms)
+        err, err0 = blockwise(outs[0], ref), blockwise(outs[1], ref)
+        e0 = [blockwise(o[:, :, z], ref[:, :, z]) for o in outs] if z else [0, 0]
+        say(f"  vs oracle: warp-specialized {err:.3e} (p=0 {e0[0]:.3e}), uniform {err0:.3e} (p=0 {e0[1]:.3e})")
+        assert err <= MIXED_TOL
+    assert diff == 0
+    assert dz <= 1e-6
