@@ -248,10 +248,9 @@ WS_CASES = [
 
 @pytest.mark.parametrize("case", range(len(WS_CASES)))
 def test_mixed_warp_specialized_matches_uniform(case):
-    """The warp-specialized FP16 stage-2 kernel runs the uniform kernel's arithmetic
-    in the same order for every p != 0: those outputs must agree bit for bit.  The
-    p = 0 sums are formed in another order (per-site sums in the stage-1 waves)
-    and must agree to FP32 rounding of the fluctuations (1e-7)."""
+    """The warp-specialized FP16 stage-2 kernel (per-column scales, zero-momentum
+    sums in the stage-1 waves) against the uniform kernel (per-tile scale) and,
+    at V3 >= 4096 where the FP16 form is used by default, against the oracle."""
     latt, Ne, nn, moms = WS_CASES[case]
     U, V = S.make_inputs_np(latt, Ne, "haar", seed=90 + case)
     say(f"ws vs uniform: latt={latt} Ne={Ne} nabla={nn} P={len(moms)}")
@@ -262,16 +261,11 @@ def test_mixed_warp_specialized_matches_uniform(case):
         g.set_option(_native.ED_OPT_MIXED_WS, ws)
         g.load_arrays(U, V)
         outs.append(g.calc_all())
-    nz = [i for i, m in enumerate(moms) if tuple(m) != (0, 0, 0)]
-    diff = np.abs(outs[0][:, :, nz] - outs[1][:, :, nz]).max()
-    z = [i for i, m in enumerate(moms) if tuple(m) == (0, 0, 0)]
-    dz = blockwise(outs[0][:, :, z], outs[1][:, :, z]) if z else 0.0
-    say(f"  max |ws - uniform| over p != 0: {diff:.1e}; p = 0 blockwise {dz:.1e}")
+    d = blockwise(outs[0], outs[1])
+    say(f"  ws vs uniform blockwise {d:.2e}")
     if latt[0] * latt[1] * latt[2] >= 4096:
         ref = O.elemental(V, U, latt, nn, moms)
         err, err0 = blockwise(outs[0], ref), blockwise(outs[1], ref)
-        e0 = [blockwise(o[:, :, z], ref[:, :, z]) for o in outs] if z else [0, 0]
-        say(f"  vs oracle: warp-specialized {err:.3e} (p=0 {e0[0]:.3e}), uniform {err0:.3e} (p=0 {e0[1]:.3e})")
+        say(f"  vs oracle: warp-specialized {err:.3e}, uniform {err0:.3e}")
         assert err <= MIXED_TOL
-    assert diff == 0
-    assert dz <= 1e-6
+    assert d <= MIXED_TOL
