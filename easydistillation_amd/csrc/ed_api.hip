@@ -522,8 +522,8 @@ void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t 
     const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
     const int child0 = level_start(k);
     launch(ED_KERNEL_STENCIL, [&] {
-      dim3 grid((c->V3 + 3) / 4, 3 * nparent, nb);
-      ed::hop_kernel<T><<<grid, 256, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0);
+      dim3 grid((c->V3 + 3) / 4, nparent, nb);
+      ed::hop3_kernel<T><<<grid, 256, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0);
     });
   }
 }
