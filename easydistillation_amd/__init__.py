@@ -8,6 +8,7 @@ Drop-in for the ``ElementalGenerator`` path of EasyDistillation
 from .backend import get_backend, set_backend
 from .constant import Nc, Nd, Ns
 from .elemental import ElementalGenerator, blending_weights
+from .displacement import DisplacementElementalGenerator
 from .insertion import MomentumPhase, derivative, derivative_index, num_derivative, output
 from .preset import (
     ArrayFileData,
@@ -26,6 +27,7 @@ __all__ = [
     "Nd",
     "Ns",
     "ElementalGenerator",
+    "DisplacementElementalGenerator",
     "blending_weights",
     "MomentumPhase",
     "derivative",

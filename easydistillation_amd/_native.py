@@ -36,6 +36,7 @@ SIGNATURES = {
     "ed_set_eigvecs": (_I, [_VP, _VP, _I, _I]),
     "ed_set_momenta": (_I, [_VP, ctypes.POINTER(ctypes.c_int32), _I]),
     "ed_set_derivatives": (_I, [_VP, _I]),
+    "ed_set_displacement": (_I, [_VP, _I]),
     "ed_set_weights": (_I, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "ed_compute": (_I, [_VP, _I, _I, _I, _VP, _I]),
     "ed_synchronize": (_I, [_VP]),
@@ -145,6 +146,9 @@ class Context:
 
     def set_derivatives(self, num_nabla: int) -> None:
         self._chk("ed_set_derivatives", self.lib.ed_set_derivatives(self.h, num_nabla))
+
+    def set_displacement(self, distance: int) -> None:
+        self._chk("ed_set_displacement", self.lib.ed_set_displacement(self.h, distance))
 
     def set_weights(self, coeff_ptr: int | None) -> None:
         p = ctypes.cast(_VP(coeff_ptr), ctypes.POINTER(ctypes.c_double)) if coeff_ptr else None

@@ -115,6 +115,14 @@ struct ed_context {
   int Lx = 0, Ly = 0, Lz = 0, Lt = 0, Ne = 0, V3 = 0, V3P = 0, NeP = 0;
   bool lattice_set = false, gauge_set = false, eig_set = false, mom_set = false;
   int num_nabla = 0, D = 1;
+  // displacement mode (displacement_elemental.py): D = distance + 1 outputs,
+  // workspace slots M_0 = V, M_1 .. M_distance, and two sets of six paths
+  bool disp = false;
+  int distance = 0;
+  int slots() const {
+    if (!disp) return D;
+    return distance + 1 + (distance >= 1 ? 6 : 0) + (distance >= 2 ? 6 : 0);
+  }
   std::vector<int32_t> moms;  // [P][3]
 
   DevBuf<double2> U64;
@@ -282,7 +290,24 @@ void build_plan(ed_context* c) {
   c->mfma_per_chunk_stage1 = 0;
   c->total_splits = 0;
   c->nks_max = 1;
-  for (int d = 0; d < D; ++d) {
+  // displacement mode: output n is the single term conj(V) . M_n
+  for (int d = 0; c->disp && d < D; ++d) {
+    DPlan p{};
+    p.d = d;
+    p.partner = -1;
+    p.sign = 1;
+    p.first = (int)sp.size();
+    p.nsplit = 1;
+    SplitTerm st{};
+    st.nL = 0;
+    st.nR = d;
+    st.coeff = 1.f;
+    sp.push_back(st);
+    c->total_splits += 1;
+    c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4;
+    dpl.push_back(p);
+  }
+  for (int d = 0; !c->disp && d < D; ++d) {
     std::vector<int> t = derivative_tuple(d);
     c->total_splits += (int64_t)1 << t.size();
     std::vector<int> r(t.rbegin(), t.rend());
@@ -419,7 +444,8 @@ void build_plan(ed_context* c) {
     std::vector<int2> bds(D);
     c->bns_max = 1;
     for (int d = 0; d < D; ++d) {
-      const std::vector<SplitTerm> terms = merged_splits(derivative_tuple(d));
+      const std::vector<SplitTerm> terms =
+          c->disp ? std::vector<SplitTerm>{SplitTerm{0, d, 1.f, 0}} : merged_splits(derivative_tuple(d));
       bds[d] = make_int2((int)bsp.size(), (int)terms.size());
       bsp.insert(bsp.end(), terms.begin(), terms.end());
       c->bns_max = std::max(c->bns_max, (int)terms.size());
@@ -438,7 +464,7 @@ void build_plan(ed_context* c) {
             pa[st * P + p] = make_double2(std::cos(ang), std::sin(ang));
           }
         }
-    c->nparent_all = level_start(c->num_nabla);
+    c->nparent_all = c->disp ? 0 : level_start(c->num_nabla);
     std::vector<int> child(3 * std::max(1, c->nparent_all), 0);
     for (int pnt = 0; pnt < c->nparent_all; ++pnt)
       for (int mu = 0; mu < 3; ++mu) {
@@ -470,7 +496,7 @@ __global__ void c64_to_c128_kernel(const float2* __restrict__ in, double2* __res
 
 template <typename T>
 size_t per_t_ws_bytes(const ed_context* c) {
-  return (size_t)c->D * c->NeP * 3 * c->V3P * sizeof(ed::c_t<T>);
+  return (size_t)c->slots() * c->NeP * 3 * c->V3P * sizeof(ed::c_t<T>);
 }
 
 // the mixed form takes the FP16-MFMA phase contraction (addressing one
@@ -510,7 +536,7 @@ int auto_tbatch(const ed_context* c, int nt) {
 // tb0 .. tb0+nb-1 into W; launch(kind, f) wraps each launch.
 template <typename T, class L>
 void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t st, L&& launch) {
-  const Geom g = c->geom(c->D);
+  const Geom g = c->geom(c->slots());
   launch(ED_KERNEL_STAGE, [&] {
     dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
     if (sizeof(T) == 8 && c->exact_v && c->v64_set)
@@ -518,6 +544,19 @@ void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t 
     else
       ed::stage_v_kernel<T, float2><<<grid, 256, 0, st>>>(c->V.p, W, g, tb0);
   });
+  if (c->disp) {
+    // paths of length n from those of length n - 1; sets A and B alternate
+    const int A = c->distance + 1, B = A + 6;
+    for (int n = 1; n <= c->distance; ++n) {
+      const int src = n == 1 ? -1 : ((n & 1) ? B : A);
+      const int dst = n == c->distance ? -1 : ((n & 1) ? A : B);
+      launch(ED_KERNEL_STENCIL, [&] {
+        dim3 grid((c->V3 + 3) / 4, (c->Ne + 63) / 64, nb);
+        ed::disp_kernel<T><<<grid, 256, 0, st>>>(W, c->U64.p, g, tb0, src, dst, n);
+      });
+    }
+    return;
+  }
   for (int k = 1; k <= c->num_nabla; ++k) {
     const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
     const int child0 = level_start(k);
@@ -530,7 +569,7 @@ void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t 
 
 ContractArgs contract_args(const ed_context* c, int nb, int t_out0) {
   ContractArgs a{};
-  a.g = c->geom(c->D);
+  a.g = c->geom(c->slots());
   a.D = c->D;
   a.P = (int)(c->moms.size() / 3);
   a.nd = c->nd;
@@ -1129,8 +1168,28 @@ int ed_set_derivatives(ed_context* ctx, int num_nabla) {
   return guard(ctx, [&] {
     ++ctx->gen;
     if (num_nabla < 0 || num_nabla > 3) fail(ED_ERR_ARG, "num_nabla must be in [0, 3]");
+    ctx->disp = false;
+    ctx->distance = 0;
     ctx->num_nabla = num_nabla;
     ctx->D = level_start(num_nabla + 1);
+    ctx->plan_dirty = true;
+    ctx->W.reset();
+    ctx->W_elem = 0;
+    ctx->stage.reset();
+    ctx->bW.reset();
+    ctx->bW_clear = false;
+  });
+}
+
+int ed_set_displacement(ed_context* ctx, int distance) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    ++ctx->gen;
+    if (distance < 0 || distance > 64) fail(ED_ERR_ARG, "distance must be in [0, 64]");
+    ctx->disp = true;
+    ctx->distance = distance;
+    ctx->num_nabla = 0;
+    ctx->D = distance + 1;
     ctx->plan_dirty = true;
     ctx->W.reset();
     ctx->W_elem = 0;
@@ -1216,6 +1275,7 @@ int ed_backward(ed_context* ctx, int t0, int nt, const void* grad_E, int grad_on
   return guard(ctx, [&] {
     require(ctx->lattice_set && ctx->gauge_set && ctx->eig_set && ctx->mom_set, ED_ERR_STATE,
             "lattice, gauge, eigenvectors and momenta must be set before ed_backward");
+    if (ctx->disp) fail(ED_ERR_UNSUPPORTED, "ed_backward covers derivative elementals, not displacement mode");
     if (!grad_E) fail(ED_ERR_ARG, "grad_E is null");
     if (!grad_V && !grad_U) fail(ED_ERR_ARG, "grad_V and grad_U are both null");
     if (t0 < 0 || nt < 1 || t0 + nt > ctx->Lt) fail(ED_ERR_ARG, "timeslice range [%d, %d) outside [0, %d)", t0, t0 + nt, ctx->Lt);
@@ -1258,7 +1318,7 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
         break;
       case ED_Q_PRIMARY_DERIVATIVES: *value = ctx->nd; break;
       case ED_Q_MOMENTUM_BLOCKS: *value = ctx->npb; break;
-      case ED_Q_HOPS_PER_T: *value = ctx->D - 1; break;
+      case ED_Q_HOPS_PER_T: *value = ctx->disp ? 6 * ctx->distance : ctx->D - 1; break;
       case ED_Q_SPLITS: *value = ctx->total_splits; break;
       default: fail(ED_ERR_ARG, "unknown query %d", what);
     }

@@ -251,6 +251,78 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
 }
 
 // --------------------------------------------------------------------------
+// Straight-path transport for displacement elementals
+// (lattice/generator/displacement_elemental.py:53-71): from the six paths of
+// length n - 1 (slots src0 + d forward, src0 + 3 + d backward; src0 < 0: all
+// six are V in slot 0) the paths of length n,
+//   forward  P_d(x)  <- U_d(x) P_d(x + d)
+//   backward P_-d(x) <- U_d(x - d)^dag P_-d(x - d),
+// are written to dst0 + d / dst0 + 3 + d (dst0 < 0: not kept) and their mean
+// over the six to slot mslot.  FP64 arithmetic, one rounding per store.
+// grid (ceil(V3 / 4), ceil(Ne / 64), TB), block 256: wave per site, lane per
+// eigenvector.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void disp_kernel(c_t<T>* __restrict__ W, const double2* __restrict__ U, Geom g,
+                                                   int tb0, int src0, int dst0, int mslot) {
+  const int s = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int e = blockIdx.y * 64 + (threadIdx.x & 63);
+  if (s >= g.V3 || e >= g.Ne) return;
+  const int tb = blockIdx.z;
+  const int x = s % g.Lx, yy = (s / g.Lx) % g.Ly, z = s / (g.Lx * g.Ly);
+  int sf[3], sb[3];
+  sf[0] = s - x + (x + 1 == g.Lx ? 0 : x + 1);
+  sb[0] = s - x + (x == 0 ? g.Lx - 1 : x - 1);
+  {
+    const int r = s - yy * g.Lx;
+    sf[1] = r + (yy + 1 == g.Ly ? 0 : yy + 1) * g.Lx;
+    sb[1] = r + (yy == 0 ? g.Ly - 1 : yy - 1) * g.Lx;
+  }
+  {
+    const int r = s - z * g.Lx * g.Ly;
+    sf[2] = r + (z + 1 == g.Lz ? 0 : z + 1) * g.Lx * g.Ly;
+    sb[2] = r + (z == 0 ? g.Lz - 1 : z - 1) * g.Lx * g.Ly;
+  }
+  c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;
+  double mr[3] = {0, 0, 0}, mi[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double2* Ud = U + ((size_t)d * g.Lt + (tb0 + tb)) * (size_t)g.V3 * 9;
+    const double2* uf = Ud + (size_t)s * 9;
+    const double2* ub = Ud + (size_t)sb[d] * 9;
+    const int slot_f = src0 < 0 ? 0 : src0 + d, slot_b = src0 < 0 ? 0 : src0 + 3 + d;
+    double2 pf[3], pb[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const c_t<T> vf = Wt[widx(g, slot_f, sf[d], b, e)], vb = Wt[widx(g, slot_b, sb[d], b, e)];
+      pf[b] = make_double2(vf.x, vf.y);
+      pb[b] = make_double2(vb.x, vb.y);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double fr = 0, fi = 0, br = 0, bi = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double2 u = uf[a * 3 + b];
+        fr += u.x * pf[b].x - u.y * pf[b].y;
+        fi += u.x * pf[b].y + u.y * pf[b].x;
+        const double2 v = ub[b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
+        br += v.x * pb[b].x + v.y * pb[b].y;
+        bi += v.x * pb[b].y - v.y * pb[b].x;
+      }
+      if (dst0 >= 0) {
+        Wt[widx(g, dst0 + d, s, a, e)] = mk<T>((T)fr, (T)fi);
+        Wt[widx(g, dst0 + 3 + d, s, a, e)] = mk<T>((T)br, (T)bi);
+      }
+      mr[a] += fr + br;
+      mi[a] += fi + bi;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) Wt[widx(g, mslot, s, a, e)] = mk<T>((T)(mr[a] / 6.0), (T)(mi[a] / 6.0));
+}
+
+// --------------------------------------------------------------------------
 // Contraction.
 // --------------------------------------------------------------------------
 struct SplitTerm {

@@ -88,6 +88,17 @@ int ed_set_momenta(ed_context* ctx, const int32_t* n, int P);
  * (elemental.py:48-49). */
 int ed_set_derivatives(ed_context* ctx, int num_nabla);
 
+/* Displacement elementals instead of derivative elementals: replaces
+ * DisplacementElementalGenerator(..., distance, ...)
+ * (lattice/generator/displacement_elemental.py:11-51, _D :53-71,
+ * calc :78-96).  Output n = 0 .. distance of ed_compute (D = distance + 1) is
+ *   E[n][p][e][f] = sum_{x,c} Phi_p(x) conj(V[e](x)_c) M_n[f](x)_c,
+ * M_0 = V and M_n the mean of the six straight paths of length n,
+ * U_mu(x) ... U_mu(x + (n-1) mu) V(x + n mu) and
+ * U_mu(x - mu)^dag ... U_mu(x - n mu)^dag V(x - n mu), mu = x, y, z.
+ * distance in [0, 64]; ed_set_derivatives returns to derivative elementals. */
+int ed_set_displacement(ed_context* ctx, int distance);
+
 /* Blending weights coeff[Ne][Ne] (float64, host), multiplied elementwise into
  * every E[d][p] (elemental.py:61-100,330-337); NULL disables blending. */
 int ed_set_weights(ed_context* ctx, const double* coeff);

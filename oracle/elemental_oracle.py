@@ -22,6 +22,10 @@ reference's elemental generation:
                         including the complex64 rounding of V (``:55,297-298``)
                         and the blending Hadamard factor (``:330-337``)
 * ``blending_coeff``    follows ``lattice/generator/elemental.py:61-100``
+* ``displacement_right`` / ``displacement_timeslice`` follow
+                        ``lattice/generator/displacement_elemental.py:53-71,78-96``
+                        (six straight paths of length n from V, their mean as
+                        the right vector, V as the left vector)
 
 Pinning (every function is checked by tests that do not retype its formula):
 ``derivative`` by the exhaustive index table and the operator names of
@@ -30,7 +34,9 @@ background closed form and gauge covariance; ``splits`` by the free-field
 closed form and the Hermiticity relations; ``elemental_timeslice`` by
 ``brute_force_timeslice`` (explicit per-site loops), the V^dag V known answer,
 translation covariance and gauge invariance; ``blending_coeff`` by a
-hand-computed table.  See ``tests/test_oracle.py``.
+hand-computed table; ``displacement_timeslice`` by ``brute_force_displacement``
+(explicit Wilson lines), the free-field closed form, the constant diagonal
+background and gauge invariance.  See ``tests/test_oracle.py``.
 
 Array layouts (reference conventions):
   U_t  [3, Lz, Ly, Lx, 3, 3]   spatial links of one timeslice, U[mu][z,y,x][a,b]
@@ -56,6 +62,10 @@ __all__ = [
     "elemental",
     "brute_force_timeslice",
     "blending_coeff",
+    "displacement_right",
+    "displacement_timeslice",
+    "displacement_elemental",
+    "brute_force_displacement",
 ]
 
 
@@ -308,4 +318,97 @@ def brute_force_timeslice(
                                     for c in range(3):
                                         s += np.conj(Lv[e, z, y, x, c]) * Rv[f, z, y, x, c]
                                     E[di, pi, e, f] += coeff * ph * s
+    return E
+
+
+# --------------------------------------------------------------------------
+# Displacement elementals (lattice/generator/displacement_elemental.py)
+# --------------------------------------------------------------------------
+def displacement_right(V: np.ndarray, U_t: np.ndarray, distance: int) -> List[np.ndarray]:
+    """Right operands M_0 .. M_distance of displacement_elemental.py:53-71.
+
+    M_0 = V; for n >= 1, M_n is the mean over the six straight paths of
+    length n: forward P_mu(x) = U_mu(x) P_mu(x + mu) and backward
+    P_-mu(x) = U_mu(x - mu)^dag P_-mu(x - mu), both started from V.
+    """
+    out = [V]
+    fwd = [V, V, V]
+    bwd = [V, V, V]
+    for n in range(1, distance + 1):
+        fwd = [np.einsum("zyxab,ezyxb->ezyxa", U_t[d], np.roll(fwd[d], -1, _axis(d))) for d in range(3)]
+        bwd = [np.roll(np.einsum("zyxba,ezyxb->ezyxa", U_t[d].conj(), bwd[d]), 1, _axis(d)) for d in range(3)]
+        out.append(sum(fwd + bwd) / 6.0)
+    return out
+
+
+def displacement_timeslice(
+    V_t: np.ndarray,
+    U_t: np.ndarray,
+    latt_size: Sequence[int],
+    distance: int,
+    momentum_list: Sequence[Sequence[int]],
+    round_v_to_c64: bool = True,
+) -> np.ndarray:
+    """E_t[n, p, e, f] = sum_x Phi_p(x) conj(V[e](x)) . M_n[f](x) (displacement_elemental.py:78-96)."""
+    V = np.asarray(V_t)
+    if round_v_to_c64:
+        V = V.astype(np.complex64)
+    V = V.astype(np.complex128)
+    Ne = V.shape[0]
+    rights = displacement_right(V, np.asarray(U_t, dtype=np.complex128), distance)
+    E = np.zeros((distance + 1, len(momentum_list), Ne, Ne), dtype=np.complex128)
+    L = V.conj()
+    for n, R in enumerate(rights):
+        Rm = R.reshape(Ne, -1)
+        for pi, m in enumerate(momentum_list):
+            ph = momentum_phase(latt_size, m)
+            E[n, pi] = (L * ph[None, :, :, :, None]).reshape(Ne, -1) @ Rm.T
+    return E
+
+
+def displacement_elemental(V, U, latt_size, distance, momentum_list):
+    """[Lt, distance + 1, P, Ne, Ne]; U [3, Lt, ...], V [Lt, Ne, ...]."""
+    return np.stack(
+        [displacement_timeslice(V[t], U[:, t], latt_size, distance, momentum_list) for t in range(V.shape[0])]
+    )
+
+
+def brute_force_displacement(V_t, U_t, latt_size, distance, momentum_list):
+    """Explicit Wilson lines per site and direction (tiny lattices only)."""
+    Lx, Ly, Lz = latt_size[0], latt_size[1], latt_size[2]
+    L = (Lx, Ly, Lz)
+    V = np.asarray(V_t).astype(np.complex64).astype(np.complex128)
+    Ne = V.shape[0]
+    E = np.zeros((distance + 1, len(momentum_list), Ne, Ne), complex)
+
+    def link(d, s):
+        return U_t[d, s[2] % Lz, s[1] % Ly, s[0] % Lx]
+
+    for z in range(Lz):
+        for y in range(Ly):
+            for x in range(Lx):
+                site = np.array([x, y, z])
+                for n in range(distance + 1):
+                    if n == 0:
+                        M = V[:, z, y, x, :]
+                    else:
+                        M = np.zeros((Ne, 3), complex)
+                        for d in range(3):
+                            step = np.eye(3, dtype=int)[d]
+                            # forward: U(x) U(x+d) ... U(x+(n-1)d) V(x+nd)
+                            W = np.eye(3, dtype=complex)
+                            for k in range(n):
+                                W = W @ link(d, site + k * step)
+                            t = (site + n * step) % L
+                            M += (W @ V[:, t[2], t[1], t[0], :].T).T
+                            # backward: U(x-d)^dag U(x-2d)^dag ... V(x-nd)
+                            W = np.eye(3, dtype=complex)
+                            for k in range(1, n + 1):
+                                W = W @ link(d, site - k * step).conj().T
+                            t = (site - n * step) % L
+                            M += (W @ V[:, t[2], t[1], t[0], :].T).T
+                        M /= 6.0
+                    for pi, m in enumerate(momentum_list):
+                        ph = np.exp(2j * np.pi * (m[0] * x / Lx + m[1] * y / Ly + m[2] * z / Lz))
+                        E[n, pi] += ph * (V[:, z, y, x, :].conj() @ M.T)
     return E

@@ -285,3 +285,78 @@ def test_fixtures_regenerate(golden_dir):
         moms = [tuple(int(c) for c in m) for m in z["momenta"]]
         E = O.elemental(z["V"], z["U"], latt, int(z["num_nabla"]), moms).transpose(1, 2, 0, 3, 4)
         assert nrm(E, z["E"]) < 1e-14
+
+
+# --------------------------------------------------------------------------
+# displacement elementals (displacement_elemental.py:53-96)
+# --------------------------------------------------------------------------
+def test_displacement_matches_brute_force():
+    latt = [3, 4, 5, 1]
+    rng = np.random.default_rng(31)
+    U = S.gauge_np(rng, latt, "haar")[:, 0]
+    V = S.eigvecs_np(rng, latt, 4)[0]
+    moms = [(0, 0, 0), (1, 0, 0), (0, -1, 2)]
+    a = O.displacement_timeslice(V, U, latt, 3, moms)
+    b = O.brute_force_displacement(V, U, latt, 3, moms)
+    assert nrm(a, b) < 1e-13
+
+
+def test_displacement_free_field_closed_form():
+    latt = [5, 4, 6, 1]
+    Lx, Ly, Lz, _ = latt
+    ks = [(1, 0, 2), (2, 3, 1), (0, 1, 5), (4, 2, 3)]
+    cs = [0, 1, 0, 2]
+    z, y, x = np.meshgrid(np.arange(Lz), np.arange(Ly), np.arange(Lx), indexing="ij")
+    V = np.zeros((len(ks), Lz, Ly, Lx, 3), complex)
+    for i, (k, c) in enumerate(zip(ks, cs)):
+        V[i, ..., c] = np.exp(2j * np.pi * (k[0] * x / Lx + k[1] * y / Ly + k[2] * z / Lz)) / np.sqrt(Lx * Ly * Lz)
+    V = V.astype(np.complex64).astype(np.complex128)
+    U = np.broadcast_to(np.eye(3, dtype=complex), (3, Lz, Ly, Lx, 3, 3)).copy()
+    moms = [(0, 0, 0), (1, 3, 1), (-1, -3, -1), (2, 0, 4)]
+    dist = 4
+    E = O.displacement_timeslice(V, U, latt, dist, moms)
+    Ls = (Lx, Ly, Lz)
+    want = np.zeros_like(E)
+    for n in range(dist + 1):
+        for pi, p in enumerate(moms):
+            for e, (ke, ce) in enumerate(zip(ks, cs)):
+                for f, (kf, cf) in enumerate(zip(ks, cs)):
+                    if ce == cf and all((ke[i] - kf[i] - p[i]) % Ls[i] == 0 for i in range(3)):
+                        lam = 1.0 if n == 0 else sum(np.cos(2 * np.pi * kf[i] * n / Ls[i]) for i in range(3)) / 3
+                        want[n, pi, e, f] = lam
+    assert np.abs(E - want).max() < 1e-6  # V rounded to complex64
+
+
+def test_displacement_constant_diagonal_background():
+    latt = [5, 4, 3, 1]
+    Lx, Ly, Lz, _ = latt
+    rng = np.random.default_rng(5)
+    phi = rng.uniform(-np.pi, np.pi, size=(3, 3))
+    U = np.zeros((3, Lz, Ly, Lx, 3, 3), complex)
+    for d in range(3):
+        U[d] = np.diag(np.exp(1j * phi[d]))
+    k, c0 = (2, 1, 1), 2
+    z, y, x = np.meshgrid(np.arange(Lz), np.arange(Ly), np.arange(Lx), indexing="ij")
+    V = np.zeros((1, Lz, Ly, Lx, 3), complex)
+    V[0, ..., c0] = np.exp(2j * np.pi * (k[0] * x / Lx + k[1] * y / Ly + k[2] * z / Lz))
+    th = [2 * np.pi * k[i] / latt[i] for i in range(3)]
+    for n, M in enumerate(O.displacement_right(V, U, 3)):
+        lam = 1.0 if n == 0 else sum(np.cos(n * (th[d] + phi[d, c0])) for d in range(3)) / 3
+        assert nrm(M, lam * V) < 1e-14
+
+
+def test_displacement_gauge_invariance():
+    latt = [3, 4, 5, 1]
+    rng = np.random.default_rng(8)
+    U = S.gauge_np(rng, latt, "haar")[:, 0]
+    V = S.eigvecs_np(rng, latt, 3)[0].astype(complex)
+    g = S.gauge_np(rng, latt, "haar")[0, 0]
+    Vg = np.einsum("zyxab,ezyxb->ezyxa", g, V)
+    Ug = np.empty_like(U)
+    for mu in range(3):
+        g_fwd = np.roll(g, -1, 2 - mu)
+        Ug[mu] = np.einsum("zyxab,zyxbc,zyxdc->zyxad", g, U[mu], g_fwd.conj())
+    moms = [(0, 0, 0), (1, 1, 0)]
+    a = O.displacement_timeslice(V, U, latt, 3, moms, round_v_to_c64=False)
+    b = O.displacement_timeslice(Vg, Ug, latt, 3, moms, round_v_to_c64=False)
+    assert nrm(b, a) < 1e-13
