@@ -14,8 +14,10 @@ from .preset import (
     ArrayFileData,
     EigenvectorHostmem,
     EigenvectorNpy,
+    EigenvectorTimeSlice,
     ElementalNpy,
     GaugeFieldHostmem,
+    GaugeFieldIldg,
     GaugeFieldNpy,
     save_elemental,
 )
@@ -37,7 +39,9 @@ __all__ = [
     "ArrayFileData",
     "EigenvectorHostmem",
     "EigenvectorNpy",
+    "EigenvectorTimeSlice",
     "ElementalNpy",
+    "GaugeFieldIldg",
     "GaugeFieldHostmem",
     "GaugeFieldNpy",
     "save_elemental",

@@ -141,14 +141,16 @@ class ElementalGenerator:
         gauge = self.gauge_field.load(key)
         U = np.asarray(gauge[:])
         Lx, Ly, Lz, Lt = self.latt_size
-        if U.shape != (Lt, Lz, Ly, Lx, Nd, Nc, Nc):
+        if U.size != Lt * Lz * Ly * Lx * Nd * Nc * Nc:
             raise ValueError(f"gauge field shape {U.shape} is not [Lt,Lz,Ly,Lx,Nd,Nc,Nc]")
-        U = U.transpose(4, 0, 1, 2, 3, 5, 6)[: Nd - 1]
+        U = U.reshape(Lt, Lz, Ly, Lx, Nd, Nc, Nc).transpose(4, 0, 1, 2, 3, 5, 6)[: Nd - 1]
         self._gauge_field_path = getattr(gauge, "file", None)
         eig = self.eigenvector.load(key)
         V = np.empty((Lt, self.Ne, Lz, Ly, Lx, Nc), dtype=np.complex64)
         for t in range(Lt):
-            V[t] = np.asarray(eig[t, : self.Ne])  # complex64 staging as elemental.py:55,297-298
+            for e in range(self.Ne):
+                # one eigenvector per read, staged in complex64 (elemental.py:55,297-298)
+                V[t, e] = np.asarray(eig[t, e]).reshape(Lz, Ly, Lx, Nc)
         self.load_arrays(U, V)
 
     def load_arrays(self, U, V) -> None:

@@ -7,9 +7,12 @@ lattice/filedata/abstract.py:12-20):
 * ``eigenvector.Ne`` and ``eigenvector.load(key)[t, e]`` -> [Lz, Ly, Lx, Nc].
 
 The classes here provide that contract for numpy ``.npy`` files (read through
-a memory map, so only the slices asked for are read) and for arrays already
-in memory.  ``ElementalNpy`` reads elementals in the reference's file layout
-[D, P, Lt, Ne, Ne] (lattice/preset.py:173-181) and ``save_elemental`` writes it.
+a memory map, so only the slices asked for are read), for arrays already in
+memory, for ILDG gauge configurations (``GaugeFieldIldg``,
+lattice/preset.py:151-160) and for QDP LazyDiskMapObj eigenvector files
+(``EigenvectorTimeSlice``, lattice/preset.py:55-64).  ``ElementalNpy`` reads
+elementals in the reference's file layout [D, P, Lt, Ne, Ne]
+(lattice/preset.py:173-181) and ``save_elemental`` writes it.
 """
 
 from __future__ import annotations
@@ -17,6 +20,8 @@ from __future__ import annotations
 from typing import List, Sequence
 
 import numpy as np
+
+from .filedata import IldgFile, IldgFileData, QDPLazyDiskMapObjFile, QDPLazyDiskMapObjFileData
 
 
 class FileMetaData:
@@ -111,6 +116,45 @@ class EigenvectorHostmem(Eigenvector):
 
     def load(self, key: str | None = None) -> ArrayFileData:
         return self.data
+
+
+class GaugeFieldIldg(IldgFile, GaugeField):
+    """ILDG (LIME) gauge configuration files ``{prefix}{key}{suffix}`` (lattice/preset.py:151-160).
+
+    ``shape`` is the binary record's shape, [Lt, Lz, Ly, Lx, Nd, Nc, Nc] for
+    elemental generation; the record is big-endian complex128 (``dtype``
+    ">c16", ILDG precision 64) or complex64 (">c8", precision 32).
+    """
+
+    def __init__(self, prefix: str, suffix: str | None = ".lime", shape: Sequence[int] = (128, 16**3, 4, 3, 3),
+                 dtype: str = ">c16"):
+        IldgFile.__init__(self)
+        self.prefix = prefix
+        self.suffix = ".lime" if suffix is None else suffix
+        self.shape = list(shape)
+        self.dtype = dtype
+
+    def load(self, key: str) -> IldgFileData:
+        return self.get_file_data(f"{self.prefix}{key}{self.suffix}", self.shape, self.dtype)
+
+
+class EigenvectorTimeSlice(QDPLazyDiskMapObjFile, Eigenvector):
+    """QDP LazyDiskMapObj eigenvector files, one record per (t, e) (lattice/preset.py:55-64).
+
+    ``shape`` = [Lt, totNe, *per-record shape], e.g. [Lt, Ne, Lz, Ly, Lx, Nc];
+    records are big-endian complex64.  ``load(key)[t, e]`` reads one
+    eigenvector.
+    """
+
+    def __init__(self, prefix: str, suffix: str | None, shape: Sequence[int] = (128, 70, 16**3, 3), totNe: int = 70):
+        QDPLazyDiskMapObjFile.__init__(self)
+        Eigenvector.__init__(self, totNe)
+        self.prefix = prefix
+        self.suffix = ".stout.n20.f0.12.laplace_eigs.3d.mod" if suffix is None else suffix
+        self.shape = list(shape)
+
+    def load(self, key: str) -> QDPLazyDiskMapObjFileData:
+        return self.get_file_data(f"{self.prefix}{key}{self.suffix}", self.shape, ">c8", 2)
 
 
 class ElementalNpy(_NpyFile):

@@ -1,0 +1,113 @@
+"""ILDG/LIME and QDP LazyDiskMapObj readers (lattice/filedata/ildg.py:76-104,
+lattice/filedata/timeslice.py:102-129) against files assembled byte by byte
+from the published layouts in this test (not by the package's writers), and
+the writers against the readers.
+"""
+
+import struct
+
+import numpy as np
+import pytest
+
+from easydistillation_amd import EigenvectorTimeSlice, GaugeFieldIldg
+from easydistillation_amd.filedata import read_lime_records, write_ildg, write_qdp_lazy
+
+
+def lime(rtype, data, flags):
+    # LIME record: magic 0x456789ab, version 1, MB/ME flags, 64-bit length, 128-byte type, data padded to 8
+    return (
+        bytes.fromhex("456789ab") + b"\x00\x01" + struct.pack(">H", flags) + struct.pack(">Q", len(data))
+        + rtype.encode().ljust(128, b"\x00") + data + b"\x00" * ((-len(data)) % 8)
+    )
+
+
+def make_lime(path, arr, precision, ns=True):
+    Lt, Lz, Ly, Lx = arr.shape[:4]
+    xmlns = ' xmlns="http://www.lqcd.org/ildg"' if ns else ""
+    xml = (f"<?xml version='1.0'?><ildgFormat{xmlns}><version>1.0</version><field>su3gauge</field>"
+           f"<precision>{precision}</precision><lx>{Lx}</lx><ly>{Ly}</ly><lz>{Lz}</lz><lt>{Lt}</lt></ildgFormat>")
+    dt = ">c16" if precision == 64 else ">c8"
+    with open(path, "wb") as f:
+        f.write(lime("ildg-format", xml.encode(), 0x8000))
+        f.write(lime("odd-length-record", b"abc", 0))  # exercises the 8-byte padding
+        f.write(lime("ildg-binary-data", arr.astype(dt).tobytes(), 0))
+        f.write(lime("ildg-data-lfn", b"lfn://x", 0x4000))
+
+
+@pytest.mark.parametrize("precision,ns", [(64, True), (32, False)])
+def test_ildg_reader_on_spec_built_file(tmp_path, precision, ns):
+    rng = np.random.default_rng(1)
+    shape = (3, 2, 4, 5, 4, 3, 3)
+    arr = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    make_lime(tmp_path / "cfg.lime", arr, precision, ns)
+    recs = read_lime_records(str(tmp_path / "cfg.lime"))
+    assert set(recs) == {"ildg-format", "odd-length-record", "ildg-binary-data", "ildg-data-lfn"}
+    g = GaugeFieldIldg(str(tmp_path) + "/", ".lime", list(shape), ">c16" if precision == 64 else ">c8")
+    data = g.load("cfg")
+    assert data.latt_size == [5, 4, 2, 3]
+    got = data[:]
+    want = arr if precision == 64 else arr.astype(np.complex64).astype(np.complex128)
+    assert got.dtype == np.complex128 and np.array_equal(got, want)
+    assert np.array_equal(data[1, 0], want[1, 0])
+
+
+def test_ildg_precision_mismatch_raises(tmp_path):
+    arr = np.zeros((1, 2, 2, 2, 4, 3, 3), complex)
+    make_lime(tmp_path / "c.lime", arr, 32)
+    with pytest.raises(ValueError):
+        GaugeFieldIldg(str(tmp_path) + "/", ".lime", list(arr.shape), ">c16").load("c")
+
+
+def test_ildg_writer_round_trip(tmp_path):
+    rng = np.random.default_rng(2)
+    shape = (2, 3, 2, 4, 4, 3, 3)
+    arr = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    write_ildg(str(tmp_path / "w.lime"), arr)
+    assert np.array_equal(GaugeFieldIldg(str(tmp_path) + "/", ".lime", list(shape)).load("w")[:], arr)
+
+
+def make_qdp(path, records, latt):
+    # length-prefixed magic, version, user XML, position of the map, records, map
+    def s(b):
+        return struct.pack(">i", len(b)) + b
+
+    xml = f"<MODMetaData><lattSize>{' '.join(map(str, latt))}</lattSize><decay_dir>3</decay_dir></MODMetaData>"
+    head = s(b"XXXXQDPLazyDiskMapObjFileXXXX") + struct.pack(">i", 1) + s(xml.encode())
+    blobs, offs, pos = [], {}, len(head) + 16
+    for k, a in records.items():
+        offs[k] = pos
+        b = a.astype(">c8").tobytes()
+        blobs.append(b)
+        pos += len(b)
+    table = struct.pack(">I", len(offs))
+    for k, o in offs.items():
+        table += struct.pack(">i", 4 * len(k)) + struct.pack(">" + "i" * len(k), *k) + struct.pack(">qq", 7, o)
+    with open(path, "wb") as f:
+        f.write(head + struct.pack(">qq", 7, pos) + b"".join(blobs) + table)
+
+
+def test_qdp_reader_on_spec_built_file(tmp_path):
+    rng = np.random.default_rng(3)
+    Lt, Ne, rec = 3, 4, (2, 3, 5, 3)
+    recs = {(t, e): (rng.normal(size=rec) + 1j * rng.normal(size=rec)) for t in range(Lt) for e in range(Ne)}
+    make_qdp(tmp_path / "e.mod", recs, [5, 3, 2, Lt])
+    ev = EigenvectorTimeSlice(str(tmp_path) + "/", ".mod", [Lt, Ne, *rec], Ne)
+    data = ev.load("e")
+    assert data.latt_size == [5, 3, 2, Lt]
+    for k, a in recs.items():
+        got = data[k]
+        assert got.dtype == np.complex64 and np.array_equal(got, a.astype(np.complex64))
+    assert np.array_equal(data[1, 2, 0], recs[(1, 2)][0].astype(np.complex64))
+    with pytest.raises(IndexError):
+        data[Lt, 0]
+    with pytest.raises(IndexError):
+        data[0, slice(None)]
+
+
+def test_qdp_writer_round_trip(tmp_path):
+    rng = np.random.default_rng(4)
+    recs = {(t, e): rng.normal(size=(4, 3)) + 0j for t in range(2) for e in range(3)}
+    write_qdp_lazy(str(tmp_path / "w.mod"), recs, [2, 2, 1, 2])
+    data = EigenvectorTimeSlice(str(tmp_path) + "/", ".mod", [2, 3, 4, 3], 3).load("w")
+    for k, a in recs.items():
+        assert np.array_equal(data[k], a.astype(np.complex64))

@@ -269,3 +269,27 @@ def test_mixed_warp_specialized_matches_uniform(case):
         say(f"  vs oracle: warp-specialized {err:.3e}, uniform {err0:.3e}")
         assert err <= MIXED_TOL
     assert d <= MIXED_TOL
+
+
+def test_load_from_ildg_and_qdp_files(tmp_path):
+    """load(key) through the reference's on-disk formats: ILDG/LIME links and
+    QDP LazyDiskMapObj eigenvectors (elemental.py:102-105, preset.py:55-64,151-160)."""
+    from easydistillation_amd.filedata import write_ildg, write_qdp_lazy
+
+    latt, Ne, nn = [4, 5, 3, 2], 7, 1
+    Lx, Ly, Lz, Lt = latt
+    moms = S.unit_momenta()
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=5)
+    full = np.empty((Lt, Lz, Ly, Lx, 4, 3, 3), np.complex128)
+    full[..., :3, :, :] = U.transpose(1, 2, 3, 4, 0, 5, 6)
+    full[..., 3, :, :] = np.eye(3)
+    write_ildg(str(tmp_path / "cfg.lime"), full)
+    write_qdp_lazy(str(tmp_path / "cfg.mod"), {(t, e): V[t, e] for t in range(Lt) for e in range(Ne)}, latt)
+    gauge = ed.GaugeFieldIldg(str(tmp_path) + "/", ".lime", [Lt, Lz, Ly, Lx, 4, 3, 3])
+    eig = ed.EigenvectorTimeSlice(str(tmp_path) + "/", ".mod", [Lt, Ne, Lz, Ly, Lx, 3], Ne)
+    g = ed.ElementalGenerator(latt, gauge, eig, nn, moms)
+    g.load("cfg")
+    E = g.calc_all()
+    err = blockwise(E, O.elemental(V, U, latt, nn, moms))
+    say(f"ILDG + QDP containers: err={err:.3e}")
+    assert err <= FP64_TOL
