@@ -47,6 +47,7 @@ SIGNATURES = {
     "ed_project_su3": (_I, [_VP, _I, ctypes.POINTER(_I)]),
     "ed_stout_smear": (_I, [_VP, _I, ctypes.c_double]),
     "ed_get_gauge": (_I, [_VP, _VP, _I]),
+    "ed_laplacian": (_I, [_VP, _I, _VP, _VP, _I, _I]),
     "ed_backward": (_I, [_VP, _I, _I, _VP, _I, _VP, _VP, _I]),
     "ed_batch_create": (_I, [_I, ctypes.POINTER(_VP)]),
     "ed_batch_destroy": (_I, [_VP]),
@@ -184,6 +185,9 @@ class Context:
 
     def stout_smear(self, nstep: int, rho: float) -> None:
         self._chk("ed_stout_smear", self.lib.ed_stout_smear(self.h, int(nstep), float(rho)))
+
+    def laplacian(self, t: int, f_ptr: int, out_ptr: int, nvec: int, on_device: bool) -> None:
+        self._chk("ed_laplacian", self.lib.ed_laplacian(self.h, int(t), _VP(f_ptr), _VP(out_ptr), int(nvec), int(on_device)))
 
     def get_gauge(self, ptr: int, on_device: bool) -> None:
         self._chk("ed_get_gauge", self.lib.ed_get_gauge(self.h, _VP(ptr), int(on_device)))

@@ -1411,6 +1411,37 @@ int ed_stout_smear(ed_context* ctx, int nstep, double rho) {
   });
 }
 
+int ed_laplacian(ed_context* ctx, int t, const void* F, void* out, int nvec, int on_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->gauge_set, ED_ERR_STATE, "ed_set_gauge must be called first");
+    if (!F || !out) fail(ED_ERR_ARG, "F and out must be non-null");
+    if (t < 0 || t >= ctx->Lt) fail(ED_ERR_ARG, "timeslice %d outside [0, %d)", t, ctx->Lt);
+    if (nvec < 1) fail(ED_ERR_ARG, "nvec must be >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)nvec * ctx->V3 * 3;
+    const double2* src = (const double2*)F;
+    double2* dst = (double2*)out;
+    DevBuf<double2> hin, hout;
+    if (!on_device) {
+      hin.ensure(n);
+      hout.ensure(n);
+      HIPCHK(hipMemcpyAsync(hin.p, F, n * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+      src = hin.p;
+      dst = hout.p;
+    }
+    const ed::gauge::Lat L{ctx->Lx, ctx->Ly, ctx->Lz, ctx->Lt};
+    const size_t threads = (size_t)nvec * ctx->V3;
+    ed::gauge::laplacian_kernel<<<(unsigned)((threads + 255) / 256), 256, 0, ctx->stream>>>(src, dst, ctx->U64.p, L, t,
+                                                                                          nvec);
+    HIPCHK(hipGetLastError());
+    if (!on_device) {
+      HIPCHK(hipMemcpyAsync(out, dst, n * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+  });
+}
+
 int ed_get_gauge(ed_context* ctx, void* U, int on_device) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {

@@ -265,5 +265,49 @@ __global__ __launch_bounds__(256) void polar_step_kernel(const double2* __restri
   store_m3(Un + 9 * i, R);
 }
 
+// 3-D gauge-covariant Laplacian on a block of vectors of timeslice t
+// (lattice/generator/eigenvector.py:11-26):
+//   (L F)(x) = 6 F(x) - sum_mu [U_mu(x) F(x + mu) + U_mu(x - mu)^dag F(x - mu)]
+// F and out [nvec][V3][3] complex128; one thread per (vector, site), sites
+// fastest so neighbouring threads read neighbouring 48-byte site vectors.
+__global__ __launch_bounds__(256) void laplacian_kernel(const double2* __restrict__ F, double2* __restrict__ out,
+                                                        const double2* __restrict__ U, Lat L, int t, int nvec) {
+  const size_t V3 = (size_t)L.Lx * L.Ly * L.Lz;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V3 * (size_t)nvec) return;
+  const size_t e = i / V3;
+  const int s = (int)(i % V3);
+  const int x = s % L.Lx, y = (s / L.Lx) % L.Ly, z = s / (L.Lx * L.Ly);
+  const double2* Fe = F + e * V3 * 3;
+  double2 acc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) acc[a] = cscale(Fe[(size_t)s * 3 + a], 6.0);
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    int fx = x, fy = y, fz = z, bx = x, by = y, bz = z;
+    step(mu, +1, fx, fy, fz, L);
+    step(mu, -1, bx, by, bz, L);
+    const size_t sf = ((size_t)fz * L.Ly + fy) * L.Lx + fx, sb = ((size_t)bz * L.Ly + by) * L.Lx + bx;
+    const double2* uf = U + 9 * L.link(mu, t, x, y, z);
+    const double2* ub = U + 9 * L.link(mu, t, bx, by, bz);
+    double2 pf[3], pb[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      pf[b] = Fe[sf * 3 + b];
+      pb[b] = Fe[sb * 3 + b];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        acc[a] = csub(acc[a], cmul(uf[a * 3 + b], pf[b]));
+        acc[a] = csub(acc[a], ccmul(ub[b * 3 + a], pb[b]));  // (U^dag)_{ab} = conj(U_{ba})
+      }
+  }
+  double2* o = out + e * V3 * 3 + (size_t)s * 3;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) o[a] = acc[a];
+}
+
 }  // namespace gauge
 }  // namespace ed

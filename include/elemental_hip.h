@@ -216,6 +216,15 @@ int ed_backward(ed_context* ctx, int t0, int nt, const void* grad_E, int grad_on
  * ed_get_gauge copies the current links out as complex128
  * U[3][Lt][Lz][Ly][Lx][3][3] (host or device pointer). */
 int ed_project_su3(ed_context* ctx, int max_iter, int* iterations);
+/* Gauge-covariant 3-D Laplacian of timeslice t on the resident links, applied
+ * to nvec vectors (the operator of EigenvectorGenerator,
+ * lattice/generator/eigenvector.py:11-26 _Laplacian, whose low modes are
+ * the eigenvectors V(t)):
+ *   out[e](x) = 6 F[e](x) - sum_mu [U_mu(x) F[e](x + mu) + U_mu(x - mu)^dagger F[e](x - mu)]
+ * F, out: [nvec][Lz][Ly][Lx][3] complex128, both host (on_device = 0) or both
+ * device pointers (the call then returns after enqueueing on the context
+ * stream). */
+int ed_laplacian(ed_context* ctx, int t, const void* F, void* out, int nvec, int on_device);
 int ed_stout_smear(ed_context* ctx, int nstep, double rho);
 int ed_get_gauge(ed_context* ctx, void* U, int on_device);
 
