@@ -30,6 +30,16 @@ METRIC = "elementals/s (t x p x d x Nev^2) mixed vs FP64 on 1 GPU at <=1e-6 rel 
 PEAK = {"fp64": 78.6, "mixed": 157.3}  # TFLOP/s dense MFMA: FP64, FP32
 PEAK_FP16 = 2500.0  # TFLOP/s dense FP16 MFMA
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per contraction launch read from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of
+# "--workload C --steps 1 --warmup 0 --calibrate-copy 4096", scaled by the factors the 4 GiB copy
+# in the same run gives (2.00 x FETCH_SIZE, 1.00 x WRITE_SIZE); the summary and the commands are in
+# profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, warp-specialized).
+TRAFFIC_MEASURED = {
+    ("C", "mixed", 1): (124143703.0 * 2.0 + 2753350.6) * 1024,
+    ("C", "fp64", 1): (284990326.9 * 2.0 + 1242298.2) * 1024,
+    ("C", "fp64", 0): (284990326.9 * 2.0 + 1242298.2) * 1024,
+}
+TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
 
 WORKLOADS = {
     # name: (latt [Lx,Ly,Lz,Lt], Ne, num_nabla, max |n|^2, links)
@@ -416,8 +426,10 @@ def main():
             "peak": PEAK[head],
             "unit": "TFLOP/s",
             "frac": r["contract_tflops_executed"] / PEAK[head],
-            "traffic": None,
-            "kernel": "contract_mixed_f16_kernel" if head == "mixed" else "contract_kernel<double>",
+            "traffic": TRAFFIC_MEASURED.get((args.workload, head, args.mixed_ws)),
+            "traffic_source": TRAFFIC_SOURCE if (args.workload, head, args.mixed_ws) in TRAFFIC_MEASURED else None,
+            "kernel": ("contract_mixed_ws_kernel<3>" if args.mixed_ws else "contract_mixed_f16_kernel<3>")
+            if head == "mixed" else "contract_kernel<double, 3>",
             "flops_basis": (
                 "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
                 "MFMA flops are converted to FP32-equivalent at the FP32:FP16 peak ratio 157.3:2500"
