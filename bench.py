@@ -35,11 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # in the same run gives (2.00 x FETCH_SIZE, 1.00 x WRITE_SIZE); the summary and the commands are in
 # profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, warp-specialized).
 TRAFFIC_MEASURED = {
-    ("C", "mixed", 1): (124143703.0 * 2.0 + 2753350.6) * 1024,
+    ("C", "mixed", 1): (124143703.0 * 2.0 + 2753350.6) * 1024,  # contract_mixed_ws_kernel<3>
     ("C", "fp64", 1): (284990326.9 * 2.0 + 1242298.2) * 1024,
     ("C", "fp64", 0): (284990326.9 * 2.0 + 1242298.2) * 1024,
 }
 TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
+MIXED_KERNEL = {2: "contract_mixed16_kernel<2>", 1: "contract_mixed_ws_kernel<3>", 0: "contract_mixed_f16_kernel<3>"}
 
 WORKLOADS = {
     # name: (latt [Lx,Ly,Lz,Lt], Ne, num_nabla, max |n|^2, links)
@@ -271,8 +272,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS) + sorted(BATCHED))
     ap.add_argument("--forms", default="mixed,fp64")
-    ap.add_argument("--mixed-ws", type=int, default=1, choices=(0, 1),
-                    help="mixed form's FP16 stage-2 kernel: 1 warp-specialized (default), 0 uniform waves")
+    ap.add_argument("--mixed-ws", type=int, default=1, choices=(0, 1, 2),
+                    help="mixed form's FP16 stage-2 kernel: 1 warp-specialized with FP32 stage 1 (default), "
+                    "2 warp-specialized with FP16 stage 1, 0 uniform waves")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument(
@@ -428,8 +430,7 @@ def main():
             "frac": r["contract_tflops_executed"] / PEAK[head],
             "traffic": TRAFFIC_MEASURED.get((args.workload, head, args.mixed_ws)),
             "traffic_source": TRAFFIC_SOURCE if (args.workload, head, args.mixed_ws) in TRAFFIC_MEASURED else None,
-            "kernel": ("contract_mixed_ws_kernel<3>" if args.mixed_ws else "contract_mixed_f16_kernel<3>")
-            if head == "mixed" else "contract_kernel<double, 3>",
+            "kernel": MIXED_KERNEL[args.mixed_ws] if head == "mixed" else "contract_kernel<double, 3>",
             "flops_basis": (
                 "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
                 "MFMA flops are converted to FP32-equivalent at the FP32:FP16 peak ratio 157.3:2500"

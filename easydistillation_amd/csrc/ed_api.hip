@@ -14,6 +14,7 @@
 
 #include "../../include/elemental_hip.h"
 #include "ed_kernels.hpp"
+#include "ed_mixed16.hpp"
 #include "ed_gauge.hpp"
 #include "ed_adjoint.hpp"
 
@@ -137,6 +138,9 @@ struct ed_context {
   int64_t mfma_per_chunk_stage1 = 0;  // sum over primary d of 16 * 4 * nks(d)
   int64_t total_splits = 0;           // split terms of elemental.py:309-321 over all d
   int nks_max = 1;                    // largest stage-1 k-step count over primary d
+  int nsplit_max = 1;                 // largest merged split-term count over primary d
+  int64_t h16_groups_sum = 0;         // sum over primary d of FP16 stage-1 k-groups
+  float cmax = 1.f;                   // largest |split coefficient|
   DevBuf<DPlan> dplan;
   DevBuf<SplitTerm> splits;
   DevBuf<int> pidx;  // plus [npb*16] then minus [npb*16]
@@ -145,7 +149,12 @@ struct ed_context {
   DevBuf<float> ph32;
   DevBuf<ed::half8> ph16;  // [npb][V3P/32][4][16][32] fp16 hi/lo of 2^14 Phi
   int mixed_stage2 = -1;   // 1: FP16 three-term split, 0: FP32 MFMA, -1: FP16 from V3 >= 4096 on
-  int mixed_ws = 1;        // FP16 stage 2: 1 warp-specialized kernel (up to 3 stage-1 k-steps), 0 uniform waves
+  // FP16 stage 2: 1 warp-specialized kernel with FP32 stage 1 (up to 3
+  // k-steps), 2 warp-specialized kernel with FP16 stage 1 (up to 2 k-groups),
+  // 0 uniform waves
+  int mixed_ws = 1;
+  float vmax = 0.f;        // largest |re|, |im| of the stored eigenvectors
+  int wsh = 0;             // scale exponent of the FP16 stage-1 workspace
   int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
   uint64_t gen = 0;        // bumped whenever buffers or plan a captured launch refers to change
 
@@ -196,6 +205,7 @@ struct ed_context {
     g.Ne = Ne;
     g.NeP = NeP;
     g.nW = nW;
+    g.wsh = 0;
     return g;
   }
 
@@ -290,6 +300,9 @@ void build_plan(ed_context* c) {
   c->mfma_per_chunk_stage1 = 0;
   c->total_splits = 0;
   c->nks_max = 1;
+  c->nsplit_max = 1;
+  c->h16_groups_sum = 0;
+  c->cmax = 1.f;
   // displacement mode: output n is the single term conj(V) . M_n
   for (int d = 0; c->disp && d < D; ++d) {
     DPlan p{};
@@ -305,6 +318,7 @@ void build_plan(ed_context* c) {
     sp.push_back(st);
     c->total_splits += 1;
     c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4;
+    c->h16_groups_sum += 1;
     dpl.push_back(p);
   }
   for (int d = 0; !c->disp && d < D; ++d) {
@@ -326,6 +340,9 @@ void build_plan(ed_context* c) {
     p.nsplit = (int)sp.size() - p.first;
     c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4 * ((3 * p.nsplit + 3) / 4);
     c->nks_max = std::max(c->nks_max, (3 * p.nsplit + 3) / 4);
+    c->nsplit_max = std::max(c->nsplit_max, p.nsplit);
+    c->h16_groups_sum += (3 * p.nsplit + 7) / 8;
+    for (int k = p.first; k < (int)sp.size(); ++k) c->cmax = std::max(c->cmax, std::fabs(sp[k].coeff));
     dpl.push_back(p);
   }
   c->nd = (int)dpl.size();
@@ -489,6 +506,15 @@ __global__ void c128_to_c64_kernel(const double2* __restrict__ in, float2* __res
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
+// largest |re|, |im| over n complex64 values into *m (bit pattern of a
+// non-negative float, so unsigned order is float order); *m starts at 0
+__global__ void absmax_c64_kernel(const float2* __restrict__ in, size_t n, unsigned int* __restrict__ m) {
+  float v = 0.f;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v = fmaxf(v, fmaxf(fabsf(in[i].x), fabsf(in[i].y)));
+  v = ed::wave_max_nonneg(v);
+  if ((threadIdx.x & 63) == 0) atomicMax(m, __float_as_uint(v));
+}
 __global__ void c64_to_c128_kernel(const float2* __restrict__ in, double2* __restrict__ out, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     out[i] = make_double2((double)in[i].x, (double)in[i].y);
@@ -504,6 +530,28 @@ size_t per_t_ws_bytes(const ed_context* c) {
 bool use_f16_stage2(const ed_context* c) {
   const bool fits32 = per_t_ws_bytes<float>(c) < ((size_t)1 << 31);
   return fits32 && (c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096));
+}
+
+// the FP16 stage-1 form of the mixed contraction: FP16 stage 2 chosen, the
+// warp-specialized FP16 kernel selected, at most two k-groups of eight
+// split-colour pairs (five split terms) per derivative, and V3 >= 4096 (on
+// smaller rough lattices the 22-bit workspace pairs push blocks that are
+// small against their terms past 1e-6; there the FP32 stage 1 runs)
+int h16_groups(const ed_context* c) { return (3 * c->nsplit_max + 7) / 8; }
+bool use_h16(const ed_context* c) {
+  return use_f16_stage2(c) && c->mixed_ws == 2 && h16_groups(c) <= 2 && c->V3 >= 4096;
+}
+
+// Scale exponent of the FP16 stage-1 workspace: every stored component,
+// times the largest split coefficient, stays below 2^15.  Unitary links keep
+// a colour vector's norm, so a hop at most doubles it and a straight path
+// keeps it; a component is at most sqrt(3) times the largest eigenvector
+// component times that growth.
+int h16_shift(const ed_context* c) {
+  const double growth = std::sqrt(3.0) * (c->disp ? 1.0 : std::ldexp(1.0, c->num_nabla));
+  const double bound = (double)c->vmax * growth * std::max(1.0f, c->cmax);
+  if (!(bound > 0)) return 0;
+  return std::max(-60, std::min(60, (int)std::floor(std::log2(32768.0 / bound))));
 }
 
 // Workspace for TB timeslices in element type T; padded rows and sites must
@@ -536,7 +584,8 @@ int auto_tbatch(const ed_context* c, int nt) {
 // tb0 .. tb0+nb-1 into W; launch(kind, f) wraps each launch.
 template <typename T, class L>
 void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t st, L&& launch) {
-  const Geom g = c->geom(c->slots());
+  Geom g = c->geom(c->slots());
+  if (std::is_same<T, ed::hsplit>::value) g.wsh = c->wsh;
   launch(ED_KERNEL_STAGE, [&] {
     dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
     if (sizeof(T) == 8 && c->exact_v && c->v64_set)
@@ -608,9 +657,22 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
 
   for (int tb0 = t0; tb0 < t0 + nt; tb0 += TB) {
     const int nb = std::min(TB, t0 + nt - tb0);
-    enqueue_stencil<T>(c, W, tb0, nb, c->stream, launch);
     ContractArgs a = contract_args(c, nb, out_dev ? (tb0 - t0) : 0);
     const int nblk = (a.nwork + 7) / 8 * 8;
+    if (sizeof(T) == 4 && use_h16(c)) {
+      // the same bytes hold the hi/lo FP16 pairs; padding stays zero in both forms
+      uint2* Wh = (uint2*)(void*)W;
+      c->wsh = h16_shift(c);
+      enqueue_stencil<ed::hsplit>(c, Wh, tb0, nb, c->stream, launch);
+      a.g.wsh = c->wsh;
+      c->launch(ED_KERNEL_CONTRACT, [&] {
+        if (h16_groups(c) <= 1)
+          ed::contract_mixed16_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wh, c->ph16.p, out, a);
+        else
+          ed::contract_mixed16_kernel<2><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wh, c->ph16.p, out, a);
+      });
+    } else {
+    enqueue_stencil<T>(c, W, tb0, nb, c->stream, launch);
     if (sizeof(T) == 4 && use_f16_stage2(c)) {
       const float2* Wf = (const float2*)(const void*)W;
       c->launch(ED_KERNEL_CONTRACT, [&] {
@@ -645,6 +707,7 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
       else
         fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", 4 * c->nks_max / 3);
     });
+    }
     if (!out_dev) {
       HIPCHK(hipMemcpyAsync((double2*)out_host + (size_t)(tb0 - t0) * out_t, out, (size_t)nb * out_t * sizeof(double2),
                             hipMemcpyDeviceToHost, c->stream));
@@ -1146,6 +1209,21 @@ int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device) {
     } else {
       ctx->V64.reset();
     }
+    {
+      // scale of the FP16 stage-1 workspace: from the largest component
+      DevBuf<unsigned int> mx;
+      mx.ensure(1);
+      HIPCHK(hipMemsetAsync(mx.p, 0, sizeof(unsigned int), ctx->stream));
+      absmax_c64_kernel<<<1024, 256, 0, ctx->stream>>>(ctx->V.p, n, mx.p);
+      HIPCHK(hipGetLastError());
+      unsigned int h = 0;
+      HIPCHK(hipMemcpyAsync(&h, mx.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      float f;
+      std::memcpy(&f, &h, sizeof(f));
+      if (!std::isfinite(f)) fail(ED_ERR_ARG, "eigenvectors hold a non-finite value");
+      ctx->vmax = f;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->eig_set = true;
   });
@@ -1237,7 +1315,9 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
         ctx->profile = value ? 1 : 0;
         break;
       case ED_OPT_MIXED_WS:
-        ctx->mixed_ws = value ? 1 : 0;
+        if (value < 0 || value > 2)
+          fail(ED_ERR_ARG, "mixed kernel kind must be 0 (uniform), 1 (FP32 stage 1) or 2 (FP16 stage 1)");
+        ctx->mixed_ws = (int)value;
         break;
       case ED_OPT_EXACT_V:
         ctx->exact_v = value ? 1 : 0;
@@ -1307,6 +1387,15 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
         // the mixed FP16 kernel: per (tile, d, 32-site chunk) stage 1 has 32 * 4 * nks(d)
         // FP32 MFMAs 16x16x4 and stage 2 has 16 N-blocks x 12 FP16 MFMAs 16x16x32
         const int64_t chunks2 = ctx->V3P / ed::SC2;
+        if (use_h16(ctx)) {
+          // FP16 stage 1: 32 sites x 4 MFMAs 16x16x32 per k-group; stage 2:
+          // 16 N-blocks x 16 MFMAs (12 momentum, 4 zero momentum)
+          *value = what == ED_Q_MIXED_F32_FLOPS_PER_T
+                       ? 0
+                       : ctx->npb * tiles * chunks2 * (32 * 4 * ctx->h16_groups_sum + (int64_t)ctx->nd * 16 * 16) * 2 *
+                             16 * 16 * 32;
+          break;
+        }
         if (what == ED_Q_MIXED_F32_FLOPS_PER_T)
           *value = ctx->npb * tiles * chunks2 * (2 * ctx->mfma_per_chunk_stage1) * 2 * 16 * 16 * 4;
         else

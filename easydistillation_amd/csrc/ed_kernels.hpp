@@ -39,6 +39,40 @@ template <typename T> using c_t = typename cplx<T>::type;
 
 template <typename T> __device__ __forceinline__ c_t<T> mk(T r, T i) { c_t<T> v; v.x = r; v.y = i; return v; }
 
+// Workspace element of the mixed form's FP16 stage 1: the complex value w,
+// scaled by 2^wsh, as FP16 hi/lo pairs, x = (hi re, hi im) and
+// y = (lo re, lo im), hi = fp16(w 2^wsh), lo = fp16(w 2^wsh - hi): 22 bits of
+// each component in 8 bytes, the size of a complex64.
+struct hsplit {};
+template <> struct cplx<hsplit> { using type = uint2; };
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+// read / write one workspace element as a complex double (wsh: the hsplit
+// scale exponent, unused by the other forms)
+__device__ __forceinline__ double2 wget(const double2 v, int) { return v; }
+__device__ __forceinline__ double2 wget(const float2 v, int) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 wget(const uint2 v, int wsh) {
+  // hi + lo needs at most 22 significant bits: the FP32 sum is exact
+  const h2v h = __builtin_bit_cast(h2v, v.x), l = __builtin_bit_cast(h2v, v.y);
+  return make_double2(ldexp((double)((float)h.x + (float)l.x), -wsh), ldexp((double)((float)h.y + (float)l.y), -wsh));
+}
+template <typename WT> __device__ __forceinline__ WT wput(double re, double im, int wsh);
+template <> __device__ __forceinline__ double2 wput<double2>(double re, double im, int) { return make_double2(re, im); }
+template <> __device__ __forceinline__ float2 wput<float2>(double re, double im, int) {
+  return make_float2((float)re, (float)im);
+}
+template <> __device__ __forceinline__ uint2 wput<uint2>(double re, double im, int wsh) {
+  // the residual is formed from the FP32 rounding of w 2^wsh (2^-24 of it,
+  // below the pair's 2^-22), exactly inside v_fma_mix
+  const float xr = (float)ldexp(re, wsh), xi = (float)ldexp(im, wsh);
+  const h2v h = {(_Float16)xr, (_Float16)xi};
+  const uint32_t hu = __builtin_bit_cast(uint32_t, h);
+  uint32_t l;
+  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hu), "v"(xr));
+  asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hu), "v"(xi));
+  return make_uint2(hu, l);
+}
+
 // --------------------------------------------------------------------------
 // MFMA wrappers: 16x16x4, one operand value per lane.
 //   A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k (both types).
@@ -66,6 +100,7 @@ struct Geom {
   int Lx, Ly, Lz, Lt;
   int V3, V3P, Ne, NeP;
   int nW;
+  int wsh;  // scale exponent of an hsplit workspace (0 otherwise)
 };
 
 // element (n, x, c, e) of one timeslice's W block
@@ -103,7 +138,7 @@ __global__ __launch_bounds__(256) void stage_v_kernel(const VT* __restrict__ V, 
     const int ml = ty * 16 + k, m = m0 + ml, e = e0 + tx;
     if (m < M && e < g.Ne) {
       const VT v = tile[tx][ml];
-      Wt[(size_t)m * g.NeP + e] = mk<T>((T)v.x, (T)v.y);
+      Wt[(size_t)m * g.NeP + e] = wput<c_t<T>>((double)v.x, (double)v.y, g.wsh);
     }
   }
 }
@@ -228,9 +263,8 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
       double2 af[3], ab[3];
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        const c_t<T> vf = Wt[widx(g, parent, sf[mu], b, e)], vb = Wt[widx(g, parent, sb[mu], b, e)];
-        af[b] = make_double2(vf.x, vf.y);
-        ab[b] = make_double2(vb.x, vb.y);
+        af[b] = wget(Wt[widx(g, parent, sf[mu], b, e)], g.wsh);
+        ab[b] = wget(Wt[widx(g, parent, sb[mu], b, e)], g.wsh);
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -244,7 +278,7 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
           re -= v.x * ab[b].x + v.y * ab[b].y;
           im -= v.x * ab[b].y - v.y * ab[b].x;
         }
-        Wo[widx(g, child, s, a, e)] = mk<T>((T)re, (T)im);
+        Wo[widx(g, child, s, a, e)] = wput<c_t<T>>(re, im, g.wsh);
       }
     }
   }
@@ -294,9 +328,8 @@ __global__ __launch_bounds__(256) void disp_kernel(c_t<T>* __restrict__ W, const
     double2 pf[3], pb[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const c_t<T> vf = Wt[widx(g, slot_f, sf[d], b, e)], vb = Wt[widx(g, slot_b, sb[d], b, e)];
-      pf[b] = make_double2(vf.x, vf.y);
-      pb[b] = make_double2(vb.x, vb.y);
+      pf[b] = wget(Wt[widx(g, slot_f, sf[d], b, e)], g.wsh);
+      pb[b] = wget(Wt[widx(g, slot_b, sb[d], b, e)], g.wsh);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -311,15 +344,15 @@ __global__ __launch_bounds__(256) void disp_kernel(c_t<T>* __restrict__ W, const
         bi += v.x * pb[b].y - v.y * pb[b].x;
       }
       if (dst0 >= 0) {
-        Wt[widx(g, dst0 + d, s, a, e)] = mk<T>((T)fr, (T)fi);
-        Wt[widx(g, dst0 + 3 + d, s, a, e)] = mk<T>((T)br, (T)bi);
+        Wt[widx(g, dst0 + d, s, a, e)] = wput<c_t<T>>(fr, fi, g.wsh);
+        Wt[widx(g, dst0 + 3 + d, s, a, e)] = wput<c_t<T>>(br, bi, g.wsh);
       }
       mr[a] += fr + br;
       mi[a] += fi + bi;
     }
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) Wt[widx(g, mslot, s, a, e)] = mk<T>((T)(mr[a] / 6.0), (T)(mi[a] / 6.0));
+  for (int a = 0; a < 3; ++a) Wt[widx(g, mslot, s, a, e)] = wput<c_t<T>>(mr[a] / 6.0, mi[a] / 6.0, g.wsh);
 }
 
 // --------------------------------------------------------------------------

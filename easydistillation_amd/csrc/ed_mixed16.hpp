@@ -1,0 +1,369 @@
+// Mixed-form contraction with both stages on FP16 matrix instructions.
+//
+// Same work decomposition, LDS layout and barrier protocol as
+// contract_mixed_ws_body (four producer waves form the Q tiles of a 32-site
+// chunk, four consumer waves contract them against the momentum phases), with
+// two changes:
+//
+// Stage 1 reads the workspace in the hsplit form (hi/lo FP16 pairs of
+// W 2^wsh) and runs on v_mfma_f32_16x16x32_f16.  Lane group g4 of k-group m
+// carries the split-colour pairs j = 8 m + 2 g4 and j + 1 (pairs beyond the
+// last are zero), as two complex FP16 pairs per operand:
+//   A  = (Lh c, Ll c, Lh' c', Ll' c')          (c: the split coefficient)
+//   B1 = (Rh, Rl, Rh', Rl')    B2 = (Rl, Rh, Rl', Rh')
+// so A.B1 + A.B2 pairs every FP16 part of L with every part of R and the dot
+// of the (re, im) pairs gives c Re(conj(L) R) up to the 22-bit split; the
+// imaginary part uses A' = ((-Li, Lr) c, ...) with the same B1, B2.  Every
+// operand register is a load or one packed multiply (the swap rides on the
+// operand selection), products of FP16 values are exact and sums FP32, as in
+// the FP32 stage 1.  FP16 matrix instructions leave the vector ALU to the
+// other wave of the SIMD, which FP32 16x16x4 instructions do not.
+//
+// The zero momentum is summed by the producers, in FP32 over their eight
+// sites of a chunk relative to each lane's chunk-0 mean, then in FP64, as in
+// contract_mixed_ws_body.
+// --------------------------------------------------------------------------
+#pragma once
+
+#include "ed_kernels.hpp"
+
+namespace ed {
+
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+// c (l.x, l.y) and c (-l.y, l.x) of a packed FP16 pair: one packed multiply
+// each, the swap done by operand selection
+__device__ __forceinline__ uint32_t pk_scale(uint32_t l, uint32_t c2) {
+  uint32_t r;
+  asm("v_pk_mul_f16 %0, %1, %2" : "=v"(r) : "v"(l), "v"(c2));
+  return r;
+}
+__device__ __forceinline__ uint32_t pk_scale_swap(uint32_t l, uint32_t cm2) {
+  uint32_t r;
+  asm("v_pk_mul_f16 %0, %1, %2 op_sel:[1,0] op_sel_hi:[0,1]" : "=v"(r) : "v"(l), "v"(cm2));
+  return r;
+}
+
+// Producer loop for exactly NG k-groups (8 split-colour pairs each).
+template <int NG>
+__device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, const ContractArgs& a, const DPlan& dp,
+                                                 int tb, int E0, int F0, float2 (*Qs)[SC2][QSITE],
+                                                 float (*wmax)[NWAVE]) {
+  typedef float4v acc_t;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l16 = lane & 15, g4 = lane >> 4;
+  const Geom g = a.g;
+  const int nchunk = g.V3P / SC2;
+  const acc_t zero = {0.f, 0.f, 0.f, 0.f};
+  const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;     // bytes per site
+  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 (checked on the host)
+  const uint2* Wt = W + (size_t)tb * (slice_b / 8);
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
+  const int J = 3 * dp.nsplit;
+  int offL[NG][2], offR[NG][2];
+  uint32_t cre[NG][2], cim[NG][2];  // (c, c) and (-c, c) as packed FP16
+#pragma unroll
+  for (int m = 0; m < NG; ++m) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = 8 * m + 2 * g4 + h;
+      // pairs beyond the last read a valid element and multiply it by 0
+      const int jj = min(j, J - 1);
+      const SplitTerm st = a.splits[dp.first + jj / 3];
+      offL[m][h] = (int)(widx(g, st.nL, 0, jj % 3, E0 + l16) * 8);
+      offR[m][h] = (int)(widx(g, st.nR, 0, jj % 3, F0 + l16) * 8);
+      const _Float16 cf = j < J ? (_Float16)st.coeff : (_Float16)0.f;
+      cre[m][h] = __builtin_bit_cast(uint32_t, h2v{cf, cf});
+      cim[m][h] = __builtin_bit_cast(uint32_t, h2v{-cf, cf});
+    }
+  }
+  uint2 lv[NG][2][SPP], rv[NG][2][SPP];
+  auto load = [&](int m, int i, int c0) {
+    const int xo = (c0 + SPP * wave + i) * (int)site_b;  // uniform
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      lv[m][h][i] = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(wr, offL[m][h], xo, 0));
+      rv[m][h][i] = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(wr, offR[m][h], xo, 0));
+    }
+  };
+
+  // zero-momentum sums of this wave's sites, rows 4 g4 + r, column l16, of
+  // Q - z0 with z0 this lane's mean over its sites of chunk 0 (added back in
+  // FP64 at the end), so the FP32 partial sums carry only fluctuations
+  double zr[4] = {0, 0, 0, 0}, zi[4] = {0, 0, 0, 0};
+  float z0r[4] = {0.f, 0.f, 0.f, 0.f}, z0i[4] = {0.f, 0.f, 0.f, 0.f};
+
+  auto stage1 = [&](int chunk, int buf, bool first) {
+    const int c0n = min(chunk + 1, nchunk - 1) * SC2;  // the last chunk re-reads itself
+    acc_t qr[SPP], qi[SPP];
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) {
+        const uint2 l0 = lv[m][0][i], l1 = lv[m][1][i], r0 = rv[m][0][i], r1 = rv[m][1][i];
+        const half8 Ar = __builtin_bit_cast(half8, u4v{pk_scale(l0.x, cre[m][0]), pk_scale(l0.y, cre[m][0]),
+                                                       pk_scale(l1.x, cre[m][1]), pk_scale(l1.y, cre[m][1])});
+        const half8 Ai = __builtin_bit_cast(half8, u4v{pk_scale_swap(l0.x, cim[m][0]), pk_scale_swap(l0.y, cim[m][0]),
+                                                       pk_scale_swap(l1.x, cim[m][1]), pk_scale_swap(l1.y, cim[m][1])});
+        const half8 B1 = __builtin_bit_cast(half8, u4v{r0.x, r0.y, r1.x, r1.y});
+        const half8 B2 = __builtin_bit_cast(half8, u4v{r0.y, r0.x, r1.y, r1.x});
+        qr[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B1, m ? qr[i] : zero, 0, 0, 0);
+        qi[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B1, m ? qi[i] : zero, 0, 0, 0);
+        qr[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B2, qr[i], 0, 0, 0);
+        qi[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B2, qi[i], 0, 0, 0);
+        load(m, i, c0n);
+      }
+    }
+    // Q to LDS, the tile maximum and the zero-momentum sums (padded sites
+    // hold Q = 0 exactly)
+    if (first) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float ar = 0.f, ai = 0.f;
+#pragma unroll
+        for (int i = 0; i < SPP; ++i) {
+          ar += qr[i][r];
+          ai += qi[i][r];
+        }
+        z0r[r] = ar * (1.f / SPP);
+        z0i[r] = ai * (1.f / SPP);
+      }
+    }
+    float mx = 0.f;
+    float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < SPP; ++i) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Qs[buf][SPP * wave + i][mfma<float>::row(g4, r) * 16 + l16] = make_float2(qr[i][r], qi[i][r]);
+        sr[r] += qr[i][r] - z0r[r];
+        si[r] += qi[i][r] - z0i[r];
+      }
+      mx = fmaxf(fmaxf(mx, fabsf(qr[i][0])), fabsf(qi[i][0]));
+      mx = fmaxf(fmaxf(mx, fabsf(qr[i][1])), fabsf(qi[i][1]));
+      mx = fmaxf(fmaxf(mx, fabsf(qr[i][2])), fabsf(qi[i][2]));
+      mx = fmaxf(fmaxf(mx, fabsf(qr[i][3])), fabsf(qi[i][3]));
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      zr[r] += (double)sr[r];
+      zi[r] += (double)si[r];
+    }
+    mx = wave_max_nonneg(mx);
+    if (lane == 0) wmax[buf][wave] = mx;
+  };
+
+  __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+  for (int m = 0; m < NG; ++m) {
+#pragma unroll
+    for (int i = 0; i < SPP; ++i) load(m, i, 0);
+  }
+  stage1(0, 0, true);
+  __syncthreads();
+  for (int chunk = 0; chunk + 1 < nchunk; ++chunk) {
+    stage1(chunk + 1, (chunk + 1) & 1, false);
+    __syncthreads();
+  }
+  __syncthreads();  // matches the consumers' last chunk
+  // every consumer has read its last chunk: LDS buffer 0 takes the four
+  // producers' zero-momentum partial sums, rows x columns, as doubles
+  double2* zs = (double2*)(void*)&Qs[0][0][0];
+  const double nsum = (double)SPP * nchunk;  // sites this wave summed, padded ones (Q = 0) included
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    zs[(wave * 16 + mfma<float>::row(g4, r)) * 16 + l16] =
+        make_double2(fma((double)z0r[r], nsum, zr[r]), fma((double)z0i[r], nsum, zi[r]));
+  __syncthreads();
+}
+
+template <int NKS>
+__device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ W, const half8* __restrict__ ph16,
+                                                      double2* __restrict__ out, const ContractArgs& a, const int w,
+                                                      float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE]) {
+  typedef float4v acc_t;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int l16 = lane & 15, g4 = lane >> 4;
+
+  int rest = w;
+  const int tf = rest % a.ntf; rest /= a.ntf;
+  const int te = rest % a.nte; rest /= a.nte;
+  const int pb = rest % a.npb; rest /= a.npb;
+  const int di = rest % a.nd;
+  const int tb = rest / a.nd;
+  const DPlan dp = a.dplan[di];
+  const int E0 = te * 16, F0 = tf * 16;
+  const Geom g = a.g;
+  const int nchunk = g.V3P / SC2;
+  const acc_t zero = {0.f, 0.f, 0.f, 0.f};
+
+  if (wave < NPROD) {
+    // producer: one branch-free instance per k-group count (NKS groups of
+    // eight split-colour pairs at most)
+    const int ng = (3 * dp.nsplit + 7) >> 3;
+    if (NKS >= 2 && ng == 2) mixed16_producer<(NKS >= 2 ? 2 : 1)>(W, a, dp, tb, E0, F0, Qs, wmax);
+    else mixed16_producer<1>(W, a, dp, tb, E0, F0, Qs, wmax);
+    return;
+  }
+
+  // ================================ consumer ================================
+  const int cw = wave - NPROD;
+  const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4;
+  const __amdgpu_buffer_rsrc_t pr_ = __builtin_amdgcn_make_buffer_rsrc((void*)PH, (short)0, nchunk * 4 * 16 * 4 * 16, 0x00020000);
+  const int phoff = (l16 * 4 + g4) * 16;
+  half8 prh, prl, pih, pil;
+  auto load_phase = [&](int chunk) {
+    const int co = chunk * 4 * 16 * 4 * 16;  // uniform
+    prh = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 0 * 1024, 0));
+    prl = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 1 * 1024, 0));
+    pih = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 2 * 1024, 0));
+    pil = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 3 * 1024, 0));
+  };
+  double D1[NBC][4], D2[NBC][4], D3[NBC][4], D4[NBC][4];
+  float cr[NBC], ci[NBC];
+#pragma unroll
+  for (int b = 0; b < NBC; ++b) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
+  }
+  // stage 2 of one chunk for row b; qb points at this lane's first site
+  // (8 g4) of the chunk, row 4 cw, column l16
+  auto stage2 = [&](int b, const float2* qb, float sq, int ex) {
+    const float csr = -cr[b] * sq, csi = -ci[b] * sq;
+    uint32_t qrh[4], qrl[4], qih[4], qil[4];
+#pragma unroll
+    for (int v2 = 0; v2 < 4; ++v2) {
+      float sr[2], si[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float2 q = qb[(2 * v2 + h) * QSITE + b * 16];
+        sr[h] = fmaf(q.x, sq, csr);
+        si[h] = fmaf(q.y, sq, csi);
+      }
+      split16x2(sr[0], sr[1], qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], qih[v2], qil[v2]);
+    }
+    const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
+    const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
+    const half8 Qih = __builtin_bit_cast(half8, u4{qih[0], qih[1], qih[2], qih[3]});
+    const half8 Qil = __builtin_bit_cast(half8, u4{qil[0], qil[1], qil[2], qil[3]});
+    acc_t A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qrh, zero, 0, 0, 0);
+    acc_t A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qih, zero, 0, 0, 0);
+    acc_t A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qrh, zero, 0, 0, 0);
+    acc_t A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qih, zero, 0, 0, 0);
+    A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qrl, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qil, A2, 0, 0, 0);
+    A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pih, Qrl, A3, 0, 0, 0);
+    A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prh, Qil, A4, 0, 0, 0);
+    A1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, Qrh, A1, 0, 0, 0);
+    A2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, Qih, A2, 0, 0, 0);
+    A3 = __builtin_amdgcn_mfma_f32_16x16x32_f16(pil, Qrh, A3, 0, 0, 0);
+    A4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(prl, Qih, A4, 0, 0, 0);
+    const double inv = ldexp(1.0, ex - 28);  // 1 / (2^14 sq)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      D1[b][r] = fma((double)A1[r], inv, D1[b][r]);
+      D2[b][r] = fma((double)A2[r], inv, D2[b][r]);
+      D3[b][r] = fma((double)A3[r], inv, D3[b][r]);
+      D4[b][r] = fma((double)A4[r], inv, D4[b][r]);
+    }
+  };
+
+  load_phase(0);
+  __syncthreads();
+  {
+    const int n0 = min(SC2, g.V3);
+#pragma unroll
+    for (int b = 0; b < NBC; ++b) {
+      float sr = 0, si = 0;
+      for (int xl = 0; xl < n0; ++xl) {
+        const float2 q = Qs[0][xl][(NBC * cw + b) * 16 + l16];
+        sr += q.x;
+        si += q.y;
+      }
+      cr[b] = sr / (float)n0;
+      ci[b] = si / (float)n0;
+    }
+  }
+  for (int chunk = 0; chunk < nchunk; ++chunk) {
+    const int c0 = chunk * SC2, buf = chunk & 1;
+    float tmax = 0.f;
+#pragma unroll
+    for (int v = 0; v < NPROD; ++v) tmax = fmaxf(tmax, wmax[buf][v]);
+    int ex = 0;
+    if (tmax > 0.f) (void)frexpf(2.f * tmax, &ex);
+    const float sq = ldexpf(1.f, 14 - ex);
+    const float2* qb = &Qs[buf][8 * g4][NBC * cw * 16 + l16];
+#pragma unroll
+    for (int b = 0; b < NBC; ++b) {
+      stage2(b, qb, sq, ex);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (chunk + 1 < nchunk) load_phase(chunk + 1);
+    __syncthreads();
+  }
+  __syncthreads();  // the producers' zero-momentum partial sums are in LDS
+  const double2* zs = (const double2*)(const void*)&Qs[0][0][0];
+
+  // ---- epilogue: values in units of 2^(2 wsh) ----
+  const double wsc = ldexp(1.0, -2 * g.wsh);
+  const int Ne = g.Ne, P = a.P;
+  const size_t plane = (size_t)Ne * Ne;
+  const size_t tbase = (size_t)(a.t_out0 + tb) * a.D * P;
+  const int f = F0 + l16;
+#pragma unroll
+  for (int b = 0; b < NBC; ++b) {
+    const int e = E0 + NBC * cw + b;
+    const bool ok = (e < Ne) && (f < Ne);
+    const double wef = ((ok && a.wgt) ? a.wgt[(size_t)e * Ne + f] : 1.0) * wsc;
+    const double wfe = ((ok && a.wgt) ? a.wgt[(size_t)f * Ne + e] : 1.0) * wsc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int bi = pb * 16 + mfma<float>::row(g4, r);
+      if (!ok || bi >= a.nbase) continue;
+      const int ip = a.plus_idx[bi], im = a.minus_idx[bi];
+      const double S = a.psum[bi], mr = (double)cr[b] * S, mi = (double)ci[b] * S;
+      const double epr = D1[b][r] - D2[b][r] + mr, epi = D3[b][r] + D4[b][r] + mi;
+      const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
+      if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
+      if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        if (im >= 0)
+          out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
+              make_double2(s * epr * wfe, -s * epi * wfe);
+        if (ip >= 0)
+          out[(tbase + (size_t)dp.partner * P + ip) * plane + (size_t)f * Ne + e] =
+              make_double2(s * emr * wfe, -s * emi * wfe);
+      }
+    }
+    // zero momentum: the producers' sums of Q over their sites
+    const int row = NBC * cw + b;
+    const double2 z0 = zs[(0 * 16 + row) * 16 + l16], z1 = zs[(1 * 16 + row) * 16 + l16];
+    const double2 z2 = zs[(2 * 16 + row) * 16 + l16], z3 = zs[(3 * 16 + row) * 16 + l16];
+    const double zre = (z0.x + z1.x) + (z2.x + z3.x), zim = (z0.y + z1.y) + (z2.y + z3.y);
+    if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
+      out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
+      if (dp.partner >= 0) {
+        const double s = (double)dp.sign;
+        out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
+            make_double2(s * zre * wfe, -s * zim * wfe);
+      }
+    }
+  }
+}
+
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed16_kernel(const uint2* __restrict__ W,
+                                                                          const half8* __restrict__ ph16,
+                                                                          double2* __restrict__ out,
+                                                                          ContractArgs a) {
+  __shared__ float2 Qs[2][SC2][QSITE];
+  __shared__ float wmax[2][NWAVE];
+  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  if (w >= a.nwork) return;
+  contract_mixed16_body<NKS>(W, ph16, out, a, w, Qs, wmax);
+}
+
+}  // namespace ed

@@ -130,8 +130,15 @@ class EigenvectorGenerator:
         F = F.contiguous()
         out = torch.empty_like(F)
         nvec = F.shape[0]
+        # the library's stream starts after the caller's stream has produced F
+        # and the caller's stream continues after the library has written out
+        caller = torch.cuda.current_stream(F.device)
+        self._stream.wait_stream(caller)
         with torch.cuda.stream(self._stream):
             self._ctx.laplacian(t, F.data_ptr(), out.data_ptr(), nvec, True)
+        F.record_stream(self._stream)
+        out.record_stream(self._stream)
+        caller.wait_stream(self._stream)
         return out
 
     # ------------------------------------------------------------- solver

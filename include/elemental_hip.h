@@ -132,8 +132,12 @@ int ed_synchronize(ed_context* ctx);
  *                         in the reference.
  *   ED_OPT_MIXED_WS       FP16 stage 2 kernel: 1 (default) warp-specialized (4 waves run the FP32
  *                         split sum, 4 the FP16 phase contraction, overlapping on each SIMD),
- *                         0 every wave runs both stages; derivatives with more than 12
- *                         (split, colour) terms always use the latter. */
+ *                         2 warp-specialized with the split sum on FP16 MFMA too (the
+ *                         workspace then holds FP16 hi/lo pairs of W 2^k, k chosen from the
+ *                         largest eigenvector component; up to 16 (split, colour) terms and
+ *                         V3 >= 4096, else kind 1 runs),
+ *                         0 every wave runs both stages; derivatives with more (split,
+ *                         colour) terms than the chosen kernel takes use the latter. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
