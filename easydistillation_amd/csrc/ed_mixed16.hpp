@@ -44,10 +44,23 @@ __device__ __forceinline__ uint32_t pk_scale_swap(uint32_t l, uint32_t cm2) {
   return r;
 }
 
+// (v.y, v.x): one 64-bit move with operand selection
+__device__ __forceinline__ uint64_t swap_pair(uint64_t v) {
+  uint64_t r;
+  asm("v_pk_mov_b32 %0, %1, %1 op_sel:[1,0]" : "=v"(r) : "v"(v));
+  return r;
+}
+typedef uint64_t u2l __attribute__((ext_vector_type(2)));
+
+// The Q tiles live in LDS as (re, im) float pairs; pair (buf, site, col)
+// starts at float qoff(buf, site, col).
+__device__ __forceinline__ int qoff(int buf, int site, int col) { return 2 * ((buf * SC2 + site) * QSITE + col); }
+constexpr int QFLOATS = 2 * 2 * SC2 * QSITE + 2;
+
 // Producer loop for exactly NG k-groups (8 split-colour pairs each).
 template <int NG>
 __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, const ContractArgs& a, const DPlan& dp,
-                                                 int tb, int E0, int F0, float2 (*Qs)[SC2][QSITE],
+                                                 int tb, int E0, int F0, float* __restrict__ Qf,
                                                  float (*wmax)[NWAVE]) {
   typedef float4v acc_t;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -106,7 +119,8 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
         const half8 Ai = __builtin_bit_cast(half8, u4v{pk_scale_swap(l0.x, cim[m][0]), pk_scale_swap(l0.y, cim[m][0]),
                                                        pk_scale_swap(l1.x, cim[m][1]), pk_scale_swap(l1.y, cim[m][1])});
         const half8 B1 = __builtin_bit_cast(half8, u4v{r0.x, r0.y, r1.x, r1.y});
-        const half8 B2 = __builtin_bit_cast(half8, u4v{r0.y, r0.x, r1.y, r1.x});
+        const half8 B2 = __builtin_bit_cast(
+            half8, u2l{swap_pair(__builtin_bit_cast(uint64_t, r0)), swap_pair(__builtin_bit_cast(uint64_t, r1))});
         qr[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B1, m ? qr[i] : zero, 0, 0, 0);
         qi[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B1, m ? qi[i] : zero, 0, 0, 0);
         qr[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B2, qr[i], 0, 0, 0);
@@ -135,7 +149,9 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
     for (int i = 0; i < SPP; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        Qs[buf][SPP * wave + i][mfma<float>::row(g4, r) * 16 + l16] = make_float2(qr[i][r], qi[i][r]);
+        const int o = qoff(buf, SPP * wave + i, mfma<float>::row(g4, r) * 16 + l16);
+        Qf[o] = qr[i][r];
+        Qf[o + 1] = qi[i][r];
         sr[r] += qr[i][r] - z0r[r];
         si[r] += qi[i][r] - z0i[r];
       }
@@ -168,7 +184,7 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
   __syncthreads();  // matches the consumers' last chunk
   // every consumer has read its last chunk: LDS buffer 0 takes the four
   // producers' zero-momentum partial sums, rows x columns, as doubles
-  double2* zs = (double2*)(void*)&Qs[0][0][0];
+  double2* zs = (double2*)(void*)Qf;
   const double nsum = (double)SPP * nchunk;  // sites this wave summed, padded ones (Q = 0) included
 #pragma unroll
   for (int r = 0; r < 4; ++r)
@@ -180,7 +196,7 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
 template <int NKS>
 __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ W, const half8* __restrict__ ph16,
                                                       double2* __restrict__ out, const ContractArgs& a, const int w,
-                                                      float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE]) {
+                                                      float* __restrict__ Qf, float (*wmax)[NWAVE]) {
   typedef float4v acc_t;
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -202,8 +218,8 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     // producer: one branch-free instance per k-group count (NKS groups of
     // eight split-colour pairs at most)
     const int ng = (3 * dp.nsplit + 7) >> 3;
-    if (NKS >= 2 && ng == 2) mixed16_producer<(NKS >= 2 ? 2 : 1)>(W, a, dp, tb, E0, F0, Qs, wmax);
-    else mixed16_producer<1>(W, a, dp, tb, E0, F0, Qs, wmax);
+    if (NKS >= 2 && ng == 2) mixed16_producer<(NKS >= 2 ? 2 : 1)>(W, a, dp, tb, E0, F0, Qf, wmax);
+    else mixed16_producer<1>(W, a, dp, tb, E0, F0, Qf, wmax);
     return;
   }
 
@@ -229,7 +245,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   }
   // stage 2 of one chunk for row b; qb points at this lane's first site
   // (8 g4) of the chunk, row 4 cw, column l16
-  auto stage2 = [&](int b, const float2* qb, float sq, int ex) {
+  auto stage2 = [&](int b, const float* qb, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
 #pragma unroll
@@ -237,7 +253,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
       float sr[2], si[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const float2 q = qb[(2 * v2 + h) * QSITE + b * 16];
+        const float2 q = ((const float2*)qb)[(2 * v2 + h) * QSITE + b * 16];
         sr[h] = fmaf(q.x, sq, csr);
         si[h] = fmaf(q.y, sq, csi);
       }
@@ -278,9 +294,9 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     for (int b = 0; b < NBC; ++b) {
       float sr = 0, si = 0;
       for (int xl = 0; xl < n0; ++xl) {
-        const float2 q = Qs[0][xl][(NBC * cw + b) * 16 + l16];
-        sr += q.x;
-        si += q.y;
+        const int o = qoff(0, xl, (NBC * cw + b) * 16 + l16);
+        sr += Qf[o];
+        si += Qf[o + 1];
       }
       cr[b] = sr / (float)n0;
       ci[b] = si / (float)n0;
@@ -294,7 +310,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     int ex = 0;
     if (tmax > 0.f) (void)frexpf(2.f * tmax, &ex);
     const float sq = ldexpf(1.f, 14 - ex);
-    const float2* qb = &Qs[buf][8 * g4][NBC * cw * 16 + l16];
+    const float* qb = Qf + qoff(buf, 8 * g4, NBC * cw * 16 + l16);
 #pragma unroll
     for (int b = 0; b < NBC; ++b) {
       stage2(b, qb, sq, ex);
@@ -304,7 +320,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     __syncthreads();
   }
   __syncthreads();  // the producers' zero-momentum partial sums are in LDS
-  const double2* zs = (const double2*)(const void*)&Qs[0][0][0];
+  const double2* zs = (const double2*)(const void*)Qf;
 
   // ---- epilogue: values in units of 2^(2 wsh) ----
   const double wsc = ldexp(1.0, -2 * g.wsh);
@@ -359,11 +375,11 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed16_kernel(const u
                                                                           const half8* __restrict__ ph16,
                                                                           double2* __restrict__ out,
                                                                           ContractArgs a) {
-  __shared__ float2 Qs[2][SC2][QSITE];
+  __shared__ __align__(16) float Qf[QFLOATS];
   __shared__ float wmax[2][NWAVE];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
-  contract_mixed16_body<NKS>(W, ph16, out, a, w, Qs, wmax);
+  contract_mixed16_body<NKS>(W, ph16, out, a, w, Qf, wmax);
 }
 
 }  // namespace ed
