@@ -80,11 +80,13 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int j = 8 * m + 2 * g4 + h;
-      // pairs beyond the last read a valid element and multiply it by 0
+      // pairs beyond the last point past the buffer's range (offset >= 2^31 >
+      // the slice size; offset + site offset < 2^32): the loads return 0
+      // without touching memory, and the coefficient is 0 as well
       const int jj = min(j, J - 1);
       const SplitTerm st = a.splits[dp.first + jj / 3];
-      offL[m][h] = (int)(widx(g, st.nL, 0, jj % 3, E0 + l16) * 8);
-      offR[m][h] = (int)(widx(g, st.nR, 0, jj % 3, F0 + l16) * 8);
+      offL[m][h] = j < J ? (int)(widx(g, st.nL, 0, jj % 3, E0 + l16) * 8) : 0x7FFFFFF0;
+      offR[m][h] = j < J ? (int)(widx(g, st.nR, 0, jj % 3, F0 + l16) * 8) : 0x7FFFFFF0;
       const _Float16 cf = j < J ? (_Float16)st.coeff : (_Float16)0.f;
       cre[m][h] = __builtin_bit_cast(uint32_t, h2v{cf, cf});
       cim[m][h] = __builtin_bit_cast(uint32_t, h2v{-cf, cf});
