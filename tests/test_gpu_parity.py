@@ -237,12 +237,15 @@ def test_mixed_stage2_variants(stage2):
 
 
 WS_CASES = [
-    # latt, Ne, num_nabla, momenta: padded sites (V3 % 32 != 0), Ne % 16 != 0,
-    # two momentum blocks, every split count up to 12 (split, colour) terms
-    ([16, 16, 16, 2], 24, 2, S.momenta_by_norm(4)),
-    ([5, 7, 9, 2], 37, 2, S.momenta_by_norm(5)[:57]),
-    ([6, 6, 6, 3], 16, 1, S.unit_momenta()),
-    ([9, 4, 3, 1], 9, 0, [(1, 0, 0), (0, 0, 0), (-1, 0, 0)]),
+    # latt, Ne, num_nabla, momenta, links: padded sites (V3 % 32 != 0),
+    # Ne % 16 != 0, two momentum blocks, every split count up to 12 (split,
+    # colour) terms, rough and smooth links at V3 >= 4096 (where the FP16
+    # stage-1 kernel runs)
+    ([16, 16, 16, 2], 24, 2, S.momenta_by_norm(4), "haar"),
+    ([5, 7, 9, 2], 37, 2, S.momenta_by_norm(5)[:57], "haar"),
+    ([6, 6, 6, 3], 16, 1, S.unit_momenta(), "haar"),
+    ([9, 4, 3, 1], 9, 0, [(1, 0, 0), (0, 0, 0), (-1, 0, 0)], "haar"),
+    ([16, 16, 24, 1], 40, 2, S.momenta_by_norm(3), "smooth"),
 ]
 
 
@@ -251,9 +254,9 @@ def test_mixed_warp_specialized_matches_uniform(case):
     """The warp-specialized FP16 stage-2 kernels (FP16 stage 1 with hi/lo
     workspace pairs; FP32 stage 1) and the uniform kernel (per-tile scale),
     each against the oracle, and against each other within twice the bound."""
-    latt, Ne, nn, moms = WS_CASES[case]
-    U, V = S.make_inputs_np(latt, Ne, "haar", seed=90 + case)
-    say(f"ws vs uniform: latt={latt} Ne={Ne} nabla={nn} P={len(moms)}")
+    latt, Ne, nn, moms, links = WS_CASES[case]
+    U, V = S.make_inputs_np(latt, Ne, links, seed=90 + case)
+    say(f"ws vs uniform: latt={latt} Ne={Ne} nabla={nn} P={len(moms)} links={links}")
     outs = {}
     for ws in (2, 1, 0):
         g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
