@@ -112,7 +112,7 @@ def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
         t_hop = time.perf_counter() - t0
         if n_h >= 2 and t_hop > seconds_target / 2:
             break
-        if n_h >= 8:
+        if n_h >= 16:
             break
     # contractions
     t_con, n_c = 0.0, 0
@@ -123,7 +123,7 @@ def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
         t_con = time.perf_counter() - t0
         if n_c >= 2 and t_con > seconds_target / 2:
             break
-        if n_c >= 8:
+        if n_c >= 16:
             break
     per_t = n_hops * (t_hop / n_h) + n_splits * len(moms) * (t_con / n_c)
     elementals_t = len(moms) * len(dl) * Ne * Ne
