@@ -135,11 +135,14 @@ struct ed_context {
   bool plan_dirty = true;
   int hermitian = 1;
   int nd = 0, npb = 1, nbase = 0, zero_idx = -1;
-  int64_t mfma_per_chunk_stage1 = 0;  // sum over primary d of 16 * 4 * nks(d)
+  // per-plan MFMA counts for the flop accounting, [0] over derivatives that
+  // run every (e, f) tile, [1] over palindromic ones (upper-triangle tiles)
+  int64_t mfma_per_chunk_stage1[2] = {0, 0};  // sum of 16 * 4 * nks(d)
+  int64_t nd_tri[2] = {0, 0};                 // derivative counts
   int64_t total_splits = 0;           // split terms of elemental.py:309-321 over all d
   int nks_max = 1;                    // largest stage-1 k-step count over primary d
   int nsplit_max = 1;                 // largest merged split-term count over primary d
-  int64_t h16_groups_sum = 0;         // sum over primary d of FP16 stage-1 k-groups
+  int64_t h16_groups_sum[2] = {0, 0};         // sum of FP16 stage-1 k-groups
   float cmax = 1.f;                   // largest |split coefficient|
   DevBuf<DPlan> dplan;
   DevBuf<SplitTerm> splits;
@@ -297,11 +300,12 @@ void build_plan(ed_context* c) {
   // derivative plan: primary d and the partner rev(d) derived by Hermiticity
   std::vector<DPlan> dpl;
   std::vector<SplitTerm> sp;
-  c->mfma_per_chunk_stage1 = 0;
+  c->mfma_per_chunk_stage1[0] = c->mfma_per_chunk_stage1[1] = 0;
+  c->nd_tri[0] = c->nd_tri[1] = 0;
   c->total_splits = 0;
   c->nks_max = 1;
   c->nsplit_max = 1;
-  c->h16_groups_sum = 0;
+  c->h16_groups_sum[0] = c->h16_groups_sum[1] = 0;
   c->cmax = 1.f;
   // displacement mode: output n is the single term conj(V) . M_n
   for (int d = 0; c->disp && d < D; ++d) {
@@ -317,8 +321,9 @@ void build_plan(ed_context* c) {
     st.coeff = 1.f;
     sp.push_back(st);
     c->total_splits += 1;
-    c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4;
-    c->h16_groups_sum += 1;
+    c->mfma_per_chunk_stage1[0] += (int64_t)ed::SC * 4;
+    c->h16_groups_sum[0] += 1;
+    c->nd_tri[0] += 1;
     dpl.push_back(p);
   }
   for (int d = 0; !c->disp && d < D; ++d) {
@@ -326,22 +331,29 @@ void build_plan(ed_context* c) {
     c->total_splits += (int64_t)1 << t.size();
     std::vector<int> r(t.rbegin(), t.rend());
     const int rd = derivative_index(r);
-    int partner = -1;
+    int partner = -1, mirror = 0;
     if (c->hermitian && rd != d) {
       if (rd < d) continue;  // written by its partner
       partner = rd;
+    } else if (c->hermitian) {
+      // palindromic d: E_d[p] = (-1)^{|d|} E_d[-p]^dagger, so tiles below the
+      // (e, f) diagonal are the transposes of those above it
+      partner = d;
+      mirror = 1;
     }
     DPlan p{};
     p.d = d;
     p.partner = partner;
+    p.mirror = mirror;
     p.sign = (t.size() % 2) ? -1 : 1;
     p.first = (int)sp.size();
     for (const SplitTerm& st : merged_splits(t)) sp.push_back(st);
     p.nsplit = (int)sp.size() - p.first;
-    c->mfma_per_chunk_stage1 += (int64_t)ed::SC * 4 * ((3 * p.nsplit + 3) / 4);
+    c->mfma_per_chunk_stage1[mirror] += (int64_t)ed::SC * 4 * ((3 * p.nsplit + 3) / 4);
+    c->nd_tri[mirror] += 1;
     c->nks_max = std::max(c->nks_max, (3 * p.nsplit + 3) / 4);
     c->nsplit_max = std::max(c->nsplit_max, p.nsplit);
-    c->h16_groups_sum += (3 * p.nsplit + 7) / 8;
+    c->h16_groups_sum[mirror] += (3 * p.nsplit + 7) / 8;
     for (int k = p.first; k < (int)sp.size(); ++k) c->cmax = std::max(c->cmax, std::fabs(sp[k].coeff));
     dpl.push_back(p);
   }
@@ -1371,14 +1383,17 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
   return guard(ctx, [&] {
     if (!value) fail(ED_ERR_ARG, "value is null");
     build_plan(ctx);
-    const int64_t tiles = (int64_t)(ctx->NeP / 16) * (ctx->NeP / 16);
+    // work items per (t, momentum block): every tile for most derivatives,
+    // the upper triangle for palindromic ones
+    const int64_t nt = ctx->NeP / 16, tiles[2] = {nt * nt, nt * (nt + 1) / 2};
+    auto tw = [&](const int64_t (&v)[2]) { return tiles[0] * v[0] + tiles[1] * v[1]; };
     const int64_t chunks = ctx->V3P / ed::SC;
     const int64_t P = (int64_t)(ctx->moms.size() / 3);
     switch (what) {
       case ED_Q_CONTRACT_FLOPS_PER_T: {
         // per work item and chunk: stage 1 (16 sites x 4 MFMAs per k-step) and
         // stage 2 (4 waves x 4 k-steps x 4 N-blocks x 4 real products)
-        const int64_t mf = ctx->npb * tiles * chunks * (ctx->mfma_per_chunk_stage1 + (int64_t)ctx->nd * 256);
+        const int64_t mf = ctx->npb * chunks * (tw(ctx->mfma_per_chunk_stage1) + tw(ctx->nd_tri) * 256);
         *value = mf * 2 * 16 * 16 * 4;
         break;
       }
@@ -1389,17 +1404,17 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
         const int64_t chunks2 = ctx->V3P / ed::SC2;
         if (use_h16(ctx)) {
           // FP16 stage 1: 32 sites x 4 MFMAs 16x16x32 per k-group; stage 2:
-          // 16 N-blocks x 16 MFMAs (12 momentum, 4 zero momentum)
+          // 16 N-blocks x 12 MFMAs
           *value = what == ED_Q_MIXED_F32_FLOPS_PER_T
                        ? 0
-                       : ctx->npb * tiles * chunks2 * (32 * 4 * ctx->h16_groups_sum + (int64_t)ctx->nd * 16 * 16) * 2 *
-                             16 * 16 * 32;
+                       : ctx->npb * chunks2 * (32 * 4 * tw(ctx->h16_groups_sum) + tw(ctx->nd_tri) * 16 * 12) * 2 * 16 *
+                             16 * 32;
           break;
         }
         if (what == ED_Q_MIXED_F32_FLOPS_PER_T)
-          *value = ctx->npb * tiles * chunks2 * (2 * ctx->mfma_per_chunk_stage1) * 2 * 16 * 16 * 4;
+          *value = ctx->npb * chunks2 * (2 * tw(ctx->mfma_per_chunk_stage1)) * 2 * 16 * 16 * 4;
         else
-          *value = ctx->npb * tiles * chunks2 * (int64_t)ctx->nd * 16 * 12 * 2 * 16 * 16 * 32;
+          *value = ctx->npb * chunks2 * tw(ctx->nd_tri) * 16 * 12 * 2 * 16 * 16 * 32;
         break;
       }
       case ED_Q_ALGO_FLOPS_PER_T:

@@ -369,8 +369,17 @@ struct DPlan {
   int partner;      // rev(d) written from this d's result, or -1
   int sign;         // (-1)^{|d|} for the partner
   int first, nsplit;
-  int pad[3];
+  int mirror;       // rev(d) = d: only tiles te <= tf run; tile (te, tf) also
+                    // writes the transposed tile (tf, te) through the partner rule
+  int pad[2];
 };
+
+// A tile of a self-partnered derivative below the diagonal is written by its
+// mirror tile; the diagonal tile computes every (e, f) itself.
+__device__ __forceinline__ bool tile_skipped(const DPlan& dp, int te, int tf) { return dp.mirror && te > tf; }
+__device__ __forceinline__ bool partner_writes(const DPlan& dp, int te, int tf) {
+  return dp.partner >= 0 && !(dp.mirror && te == tf);
+}
 
 struct ContractArgs {
   Geom g;
@@ -429,6 +438,7 @@ __device__ __forceinline__ void contract_body(const c_t<T>* __restrict__ W, cons
   const int di = rest % a.nd;
   const int tb = rest / a.nd;
   const DPlan dp = a.dplan[di];
+  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const size_t site = (size_t)3 * g.NeP;  // elements per site
@@ -653,7 +663,7 @@ __device__ __forceinline__ void contract_body(const c_t<T>* __restrict__ W, cons
       const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;  // E_d[-b]
       if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
       if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         // E_{rev d}[p'] = sign * E_d[-p']^dagger
         const double s = (double)dp.sign;
         if (im >= 0)
@@ -674,7 +684,7 @@ __device__ __forceinline__ void contract_body(const c_t<T>* __restrict__ W, cons
     zim += (double)ci[b] * g.V3;
     if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
       out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
             make_double2(s * zre * wfe, -s * zim * wfe);
@@ -754,6 +764,7 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
   const int di = rest % a.nd;
   const int tb = rest / a.nd;
   const DPlan dp = a.dplan[di];
+  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;  // bytes per site
@@ -992,7 +1003,7 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
       const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
       if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
       if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         if (im >= 0)
           out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
@@ -1011,7 +1022,7 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
     zim += (double)ci[b] * g.V3P;
     if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
       out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
             make_double2(s * zre * wfe, -s * zim * wfe);
@@ -1215,6 +1226,7 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
   const int di = rest % a.nd;
   const int tb = rest / a.nd;
   const DPlan dp = a.dplan[di];
+  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const int nchunk = g.V3P / SC2;
@@ -1353,7 +1365,7 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
       const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
       if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
       if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         if (im >= 0)
           out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
@@ -1371,7 +1383,7 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
     const double zre = (z0.x + z1.x) + (z2.x + z3.x), zim = (z0.y + z1.y) + (z2.y + z3.y);
     if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
       out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
             make_double2(s * zre * wfe, -s * zim * wfe);

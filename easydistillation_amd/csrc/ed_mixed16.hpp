@@ -211,6 +211,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   const int di = rest % a.nd;
   const int tb = rest / a.nd;
   const DPlan dp = a.dplan[di];
+  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const int nchunk = g.V3P / SC2;
@@ -346,7 +347,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
       const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
       if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
       if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         if (im >= 0)
           out[(tbase + (size_t)dp.partner * P + im) * plane + (size_t)f * Ne + e] =
@@ -363,7 +364,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     const double zre = (z0.x + z1.x) + (z2.x + z3.x), zim = (z0.y + z1.y) + (z2.y + z3.y);
     if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
       out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
-      if (dp.partner >= 0) {
+      if (partner_writes(dp, te, tf)) {
         const double s = (double)dp.sign;
         out[(tbase + (size_t)dp.partner * P + a.zero_idx) * plane + (size_t)f * Ne + e] =
             make_double2(s * zre * wfe, -s * zim * wfe);
