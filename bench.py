@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # in the same run gives (2.00 x FETCH_SIZE, 1.00 x WRITE_SIZE); the summary and the commands are in
 # profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, warp-specialized).
 TRAFFIC_MEASURED = {
-    ("C", "mixed", 1): (124143703.0 * 2.0 + 2753350.6) * 1024,  # contract_mixed_ws_kernel<3>
-    ("C", "fp64", 1): (284990326.9 * 2.0 + 1242298.2) * 1024,
-    ("C", "fp64", 0): (284990326.9 * 2.0 + 1242298.2) * 1024,
+    ("C", "mixed", 1): (88763754.9 * 2.0 + 3398472.2) * 1024,  # contract_mixed_ws_kernel<3>
+    ("C", "fp64", 1): (207139863.4 * 2.0 + 1411081.7) * 1024,
+    ("C", "fp64", 0): (207139863.4 * 2.0 + 1411081.7) * 1024,
 }
 TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
 MIXED_KERNEL = {2: "contract_mixed16_kernel<2>", 1: "contract_mixed_ws_kernel<3>", 0: "contract_mixed_f16_kernel<3>"}
