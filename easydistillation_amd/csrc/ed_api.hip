@@ -141,6 +141,7 @@ struct ed_context {
   int64_t nd_tri[2] = {0, 0};                 // derivative counts
   int64_t total_splits = 0;           // split terms of elemental.py:309-321 over all d
   int nks_max = 1;                    // largest stage-1 k-step count over primary d
+  int items_t = 0;                    // contraction work items per timeslice
   int nsplit_max = 1;                 // largest merged split-term count over primary d
   int64_t h16_groups_sum[2] = {0, 0};         // sum of FP16 stage-1 k-groups
   float cmax = 1.f;                   // largest |split coefficient|
@@ -448,6 +449,16 @@ void build_plan(ed_context* c) {
   for (int i = 0; i < c->nbase; ++i)
     if (base[i].n[0] % c->Lx == 0 && base[i].n[1] % c->Ly == 0 && base[i].n[2] % c->Lz == 0) ps[i] = (double)c->V3;
 
+  // work items of each derivative within a timeslice: npb blocks of its tiles
+  {
+    const int nt = c->NeP / 16;
+    int items = 0;
+    for (DPlan& p : dpl) {
+      p.item0 = items;
+      items += c->npb * (p.mirror ? nt * (nt + 1) / 2 : nt * nt);
+    }
+    c->items_t = items;
+  }
   c->dplan.ensure(dpl.size());
   c->splits.ensure(std::max<size_t>(1, sp.size()));
   c->pidx.ensure(idx.size());
@@ -638,7 +649,8 @@ ContractArgs contract_args(const ed_context* c, int nb, int t_out0) {
   a.nbase = c->nbase;
   a.nte = c->NeP / 16;
   a.ntf = c->NeP / 16;
-  a.nwork = nb * a.nd * a.npb * a.nte * a.ntf;
+  a.items_t = c->items_t;
+  a.nwork = nb * a.items_t;
   a.zero_idx = c->zero_idx;
   a.t_out0 = t_out0;
   a.dplan = c->dplan.p;

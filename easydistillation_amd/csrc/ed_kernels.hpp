@@ -371,12 +371,12 @@ struct DPlan {
   int first, nsplit;
   int mirror;       // rev(d) = d: only tiles te <= tf run; tile (te, tf) also
                     // writes the transposed tile (tf, te) through the partner rule
-  int pad[2];
+  int item0;        // first work item of this d within a timeslice
+  int pad;
 };
 
-// A tile of a self-partnered derivative below the diagonal is written by its
-// mirror tile; the diagonal tile computes every (e, f) itself.
-__device__ __forceinline__ bool tile_skipped(const DPlan& dp, int te, int tf) { return dp.mirror && te > tf; }
+// The diagonal tile of a self-partnered derivative computes every (e, f)
+// itself and writes no mirror.
 __device__ __forceinline__ bool partner_writes(const DPlan& dp, int te, int tf) {
   return dp.partner >= 0 && !(dp.mirror && te == tf);
 }
@@ -390,6 +390,7 @@ struct ContractArgs {
   int nte, ntf;   // e / f tiles of 16
   int nwork;      // TB * nd * npb * nte * ntf
   int zero_idx;   // output index of p = (0,0,0) handled on VALU, or -1
+  int items_t;    // work items per timeslice: sum over d of npb x (tiles of d)
   int t_out0;     // batch offset inside out (in timeslices)
   const DPlan* dplan;
   const SplitTerm* splits;
@@ -398,6 +399,36 @@ struct ContractArgs {
   const double* wgt;     // [Ne][Ne] blending weights or nullptr
   const double* psum;    // [npb*16] sum_x Phi_b(x): V3 when b = 0 mod L, else 0
 };
+
+// Work item w -> (timeslice tb, derivative di, momentum block pb, tile
+// (te, tf)).  Items of one timeslice run derivative by derivative
+// (DPlan::item0), each over npb blocks of its tiles: all nte x ntf tiles, or
+// for a self-partnered d the nte (nte + 1) / 2 with te <= tf (row-major).
+__device__ __forceinline__ void decode_item(const ContractArgs& a, int w, int& tb, int& di, int& pb, int& te,
+                                            int& tf) {
+  tb = w / a.items_t;
+  int r = w - tb * a.items_t;
+  di = 0;
+  while (di + 1 < a.nd && a.dplan[di + 1].item0 <= r) ++di;
+  r -= a.dplan[di].item0;
+  if (a.dplan[di].mirror) {
+    const int nt = a.nte, per = nt * (nt + 1) / 2;
+    pb = r / per;
+    int k = r - pb * per;
+    te = 0;
+    while (k >= nt - te) {
+      k -= nt - te;
+      ++te;
+    }
+    tf = te + k;
+  } else {
+    const int per = a.nte * a.ntf;
+    pb = r / per;
+    const int k = r - pb * per;
+    te = k / a.ntf;
+    tf = k - te * a.ntf;
+  }
+}
 
 constexpr int SC = 16;      // sites per chunk (4 k-steps of the stage-2 MFMA)
 constexpr int NWAVE = 8;    // waves per workgroup
@@ -431,14 +462,9 @@ __device__ __forceinline__ void contract_body(const c_t<T>* __restrict__ W, cons
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g4 = lane >> 4;
 
-  int rest = w;
-  const int tf = rest % a.ntf; rest /= a.ntf;
-  const int te = rest % a.nte; rest /= a.nte;
-  const int pb = rest % a.npb; rest /= a.npb;
-  const int di = rest % a.nd;
-  const int tb = rest / a.nd;
+  int tb, di, pb, te, tf;
+  decode_item(a, w, tb, di, pb, te, tf);
   const DPlan dp = a.dplan[di];
-  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const size_t site = (size_t)3 * g.NeP;  // elements per site
@@ -757,14 +783,9 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;
 
-  int rest = w;
-  const int tf = rest % a.ntf; rest /= a.ntf;
-  const int te = rest % a.nte; rest /= a.nte;
-  const int pb = rest % a.npb; rest /= a.npb;
-  const int di = rest % a.nd;
-  const int tb = rest / a.nd;
+  int tb, di, pb, te, tf;
+  decode_item(a, w, tb, di, pb, te, tf);
   const DPlan dp = a.dplan[di];
-  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;  // bytes per site
@@ -1219,14 +1240,9 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;
 
-  int rest = w;
-  const int tf = rest % a.ntf; rest /= a.ntf;
-  const int te = rest % a.nte; rest /= a.nte;
-  const int pb = rest % a.npb; rest /= a.npb;
-  const int di = rest % a.nd;
-  const int tb = rest / a.nd;
+  int tb, di, pb, te, tf;
+  decode_item(a, w, tb, di, pb, te, tf);
   const DPlan dp = a.dplan[di];
-  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const int nchunk = g.V3P / SC2;

@@ -204,14 +204,9 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;
 
-  int rest = w;
-  const int tf = rest % a.ntf; rest /= a.ntf;
-  const int te = rest % a.nte; rest /= a.nte;
-  const int pb = rest % a.npb; rest /= a.npb;
-  const int di = rest % a.nd;
-  const int tb = rest / a.nd;
+  int tb, di, pb, te, tf;
+  decode_item(a, w, tb, di, pb, te, tf);
   const DPlan dp = a.dplan[di];
-  if (tile_skipped(dp, te, tf)) return;  // uniform over the workgroup, before any barrier
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const int nchunk = g.V3P / SC2;
