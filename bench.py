@@ -275,6 +275,7 @@ def main():
     ap.add_argument("--mixed-ws", type=int, default=1, choices=(0, 1, 2),
                     help="mixed form's FP16 stage-2 kernel: 1 warp-specialized with FP32 stage 1 (default), "
                     "2 warp-specialized with FP16 stage 1, 0 uniform waves")
+    ap.add_argument("--tbatch", type=int, default=0, help="timeslices per contraction launch (0: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument(
@@ -337,6 +338,8 @@ def main():
     for form in forms:
         g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode=form)
         g.set_option(_native.ED_OPT_MIXED_WS, args.mixed_ws)
+        if args.tbatch > 0:
+            g.set_option(_native.ED_OPT_TBATCH, args.tbatch)
         g.load_arrays(U, V)
         out = torch.empty((Lt, D, P, Ne, Ne), dtype=torch.complex128, device=dev)
         for i in range(args.warmup):

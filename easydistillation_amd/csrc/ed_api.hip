@@ -179,7 +179,7 @@ struct ed_context {
 
   // workspace
   int tbatch_opt = 0;
-  int64_t ws_mb = 16384;
+  int64_t ws_mb = 36864;
   DevBuf<char> W;
   DevBuf<double2> stage;
 
@@ -598,7 +598,13 @@ int auto_tbatch(const ed_context* c, int nt) {
   int TB = c->tbatch_opt;
   if (TB <= 0) {
     const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
-    TB = (int)std::max<size_t>(1, budget / per_t_ws_bytes<T>(c));
+    const size_t per_t = per_t_ws_bytes<T>(c);
+    TB = (int)std::max<size_t>(1, budget / per_t);
+    // whole timeslices per XCD: the contraction hands each of the 8 XCDs a
+    // contiguous eighth of the work items, and the items of one timeslice
+    // differ in cost by derivative.  Large timeslices (>= 1 GiB of
+    // workspace) run eight per launch, which measured fastest at 32^3.
+    if (TB >= 8 && nt >= 8) TB = per_t >= ((size_t)1 << 30) ? 8 : (std::min(TB, nt) & ~7);
   }
   return std::min(TB, nt);
 }
