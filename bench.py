@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # in the same run gives (2.00 x FETCH_SIZE, 1.00 x WRITE_SIZE); the summary and the commands are in
 # profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, warp-specialized).
 TRAFFIC_MEASURED = {
-    ("C", "mixed", 1): (88763754.9 * 2.0 + 3398472.2) * 1024,  # contract_mixed_ws_kernel<3>
-    ("C", "fp64", 1): (207139863.4 * 2.0 + 1411081.7) * 1024,
-    ("C", "fp64", 0): (207139863.4 * 2.0 + 1411081.7) * 1024,
+    ("C", "mixed", 1): (149159766.3 * 2.0 + 3300526.8) * 1024,  # contract_mixed_ws_kernel<3>, 8 timeslices
+    ("C", "fp64", 1): (441874625.6 * 2.0 + 2777168.3) * 1024,  # contract_kernel<double, 3>, 8 timeslices
+    ("C", "fp64", 0): (441874625.6 * 2.0 + 2777168.3) * 1024,
 }
 TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
 MIXED_KERNEL = {2: "contract_mixed16_kernel<2>", 1: "contract_mixed_ws_kernel<3>", 0: "contract_mixed_f16_kernel<3>"}
@@ -403,6 +403,8 @@ def main():
 
     head = "mixed" if "mixed" in results else forms[0]
     r = results[head]
+    # the measured bytes hold for the library's own timeslice batch (8 per launch at this workload)
+    traffic_key = (args.workload, head, args.mixed_ws) if args.tbatch == 0 else None
     line = {
         "metric": METRIC,
         "value": r["elementals_per_s"],
@@ -431,8 +433,8 @@ def main():
             "peak": PEAK[head],
             "unit": "TFLOP/s",
             "frac": r["contract_tflops_executed"] / PEAK[head],
-            "traffic": TRAFFIC_MEASURED.get((args.workload, head, args.mixed_ws)),
-            "traffic_source": TRAFFIC_SOURCE if (args.workload, head, args.mixed_ws) in TRAFFIC_MEASURED else None,
+            "traffic": TRAFFIC_MEASURED.get(traffic_key),
+            "traffic_source": TRAFFIC_SOURCE if traffic_key in TRAFFIC_MEASURED else None,
             "kernel": MIXED_KERNEL[args.mixed_ws] if head == "mixed" else "contract_kernel<double, 3>",
             "flops_basis": (
                 "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
