@@ -400,10 +400,48 @@ struct ContractArgs {
   const double* psum;    // [npb*16] sum_x Phi_b(x): V3 when b = 0 mod L, else 0
 };
 
+// Tile k of one (timeslice, derivative, momentum block): all nte x ntf tiles,
+// or for a self-partnered derivative the nte (nte + 1) / 2 with te <= tf.
+// Tiles run in strips of TSTRIP e-tile rows: within a strip f-tile by f-tile,
+// and within an f-tile the strip's rows, so 32 consecutive items (one per CU
+// of an XCD, started together) read about 4 e slabs and 8 f slabs of the
+// workspace, which then hit in that XCD's L2.
+constexpr int TSTRIP = 4;
+__host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror, int& te, int& tf) {
+  int s0 = 0;
+  if (mirror) {
+    for (;;) {  // strip rows [s0, s0 + h), columns tf >= s0, te <= tf
+      const int h = min(TSTRIP, nte - s0);
+      const int cnt = h * (nte - s0) - h * (h - 1) / 2;
+      if (k < cnt) break;
+      k -= cnt;
+      s0 += TSTRIP;
+    }
+    const int h = min(TSTRIP, nte - s0);
+    tf = s0;
+    for (;;) {
+      const int c = min(h, tf - s0 + 1);
+      if (k < c) break;
+      k -= c;
+      ++tf;
+    }
+    te = s0 + k;
+  } else {
+    for (;;) {
+      const int cnt = min(TSTRIP, nte - s0) * ntf;
+      if (k < cnt) break;
+      k -= cnt;
+      s0 += TSTRIP;
+    }
+    const int h = min(TSTRIP, nte - s0);
+    tf = k / h;
+    te = s0 + (k - tf * h);
+  }
+}
+
 // Work item w -> (timeslice tb, derivative di, momentum block pb, tile
 // (te, tf)).  Items of one timeslice run derivative by derivative
-// (DPlan::item0), each over npb blocks of its tiles: all nte x ntf tiles, or
-// for a self-partnered d the nte (nte + 1) / 2 with te <= tf (row-major).
+// (DPlan::item0), each over npb blocks of its tiles (decode_tile).
 __device__ __forceinline__ void decode_item(const ContractArgs& a, int w, int& tb, int& di, int& pb, int& te,
                                             int& tf) {
   tb = w / a.items_t;
@@ -411,23 +449,10 @@ __device__ __forceinline__ void decode_item(const ContractArgs& a, int w, int& t
   di = 0;
   while (di + 1 < a.nd && a.dplan[di + 1].item0 <= r) ++di;
   r -= a.dplan[di].item0;
-  if (a.dplan[di].mirror) {
-    const int nt = a.nte, per = nt * (nt + 1) / 2;
-    pb = r / per;
-    int k = r - pb * per;
-    te = 0;
-    while (k >= nt - te) {
-      k -= nt - te;
-      ++te;
-    }
-    tf = te + k;
-  } else {
-    const int per = a.nte * a.ntf;
-    pb = r / per;
-    const int k = r - pb * per;
-    te = k / a.ntf;
-    tf = k - te * a.ntf;
-  }
+  const bool mirror = a.dplan[di].mirror;
+  const int per = mirror ? a.nte * (a.nte + 1) / 2 : a.nte * a.ntf;
+  pb = r / per;
+  decode_tile(r - pb * per, a.nte, a.ntf, mirror, te, tf);
 }
 
 constexpr int SC = 16;      // sites per chunk (4 k-steps of the stage-2 MFMA)
