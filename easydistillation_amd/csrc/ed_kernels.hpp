@@ -108,6 +108,14 @@ __device__ __forceinline__ size_t widx(const Geom& g, int n, int x, int c, int e
   return (((size_t)n * g.V3P + x) * 3 + c) * g.NeP + e;
 }
 
+// Byte offset of W[n][site 0][c][e] (8-byte elements) for a raw buffer load,
+// or oob -- at or past the end of the buffer -- for a padded eigenvector row
+// (e >= Ne): such a load returns 0, like the zero padding it stands for,
+// without a memory request.
+__device__ __forceinline__ int woff8(const Geom& g, int n, int c, int e, int oob) {
+  return e < g.Ne ? (int)(widx(g, n, 0, c, e) * 8) : oob;
+}
+
 // --------------------------------------------------------------------------
 // Stage V(t) ([Ne][V3*3], complex64 or, for exact eigenvectors, complex128)
 // into slot 0 of W ([V3P*3][NeP]) in the compute precision: a 64 x 64 tile transpose through LDS so that both the
@@ -814,14 +822,17 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;  // bytes per site
-  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^32 (checked on the host)
+  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 (the FP16 phase contraction runs only then)
   const float2* Wt = W + (size_t)tb * (slice_b / 8);
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
   const int J = __builtin_amdgcn_readfirstlane(3 * dp.nsplit);
   const int nks = (J + 3) >> 2;  // uniform
   const int nchunk = g.V3P / SC2;
 
-  // per k-step byte offsets of site 0 (within the timeslice block) and coefficients
+  // per k-step byte offsets of site 0 (within the timeslice block) and
+  // coefficients; unused (split, colour) slots and padded rows read past the
+  // buffer's end (0, no memory request)
+  const int oob = (int)slice_b;
   int offL[NKS], offR[NKS];
   float cf[NKS];
 #pragma unroll
@@ -830,11 +841,11 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
     if (ks < nks && j < J) {
       const SplitTerm st = a.splits[dp.first + j / 3];
       const int c = j % 3;
-      offL[ks] = (int)(widx(g, st.nL, 0, c, E0 + l16) * 8);
-      offR[ks] = (int)(widx(g, st.nR, 0, c, F0 + l16) * 8);
+      offL[ks] = woff8(g, st.nL, c, E0 + l16, oob);
+      offR[ks] = woff8(g, st.nR, c, F0 + l16, oob);
       cf[ks] = st.coeff;
     } else {
-      offL[ks] = offR[ks] = 0;
+      offL[ks] = offR[ks] = oob;
       cf[ks] = 0.f;
     }
   }
@@ -1134,6 +1145,9 @@ __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, 
   const float2* Wt = W + (size_t)tb * (slice_b / 8);
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
   const int J = 3 * dp.nsplit;
+  // unused (split, colour) slots and padded rows read past the buffer's end
+  // (0, no memory request)
+  const int oob = (int)slice_b;
   int offL[NK], offR[NK];
   float cf[NK];
 #pragma unroll
@@ -1142,11 +1156,11 @@ __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, 
     if (j < J) {
       const SplitTerm st = a.splits[dp.first + j / 3];
       const int c = j % 3;
-      offL[ks] = (int)(widx(g, st.nL, 0, c, E0 + l16) * 8);
-      offR[ks] = (int)(widx(g, st.nR, 0, c, F0 + l16) * 8);
+      offL[ks] = woff8(g, st.nL, c, E0 + l16, oob);
+      offR[ks] = woff8(g, st.nR, c, F0 + l16, oob);
       cf[ks] = st.coeff;
     } else {
-      offL[ks] = offR[ks] = 0;
+      offL[ks] = offR[ks] = oob;
       cf[ks] = 0.f;
     }
   }

@@ -73,6 +73,7 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
   const uint2* Wt = W + (size_t)tb * (slice_b / 8);
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
   const int J = 3 * dp.nsplit;
+  const int oob = (int)slice_b;  // past the buffer's end: the load returns 0 without a memory request
   int offL[NG][2], offR[NG][2];
   uint32_t cre[NG][2], cim[NG][2];  // (c, c) and (-c, c) as packed FP16
 #pragma unroll
@@ -80,13 +81,13 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int j = 8 * m + 2 * g4 + h;
-      // pairs beyond the last point past the buffer's range (offset >= 2^31 >
-      // the slice size; offset + site offset < 2^32): the loads return 0
-      // without touching memory, and the coefficient is 0 as well
+      // pairs beyond the last and padded rows read past the buffer's end
+      // (the slice size, < 2^31; offset + site offset < 2^32): the loads
+      // return 0 without touching memory, and the coefficient is 0 as well
       const int jj = min(j, J - 1);
       const SplitTerm st = a.splits[dp.first + jj / 3];
-      offL[m][h] = j < J ? (int)(widx(g, st.nL, 0, jj % 3, E0 + l16) * 8) : 0x7FFFFFF0;
-      offR[m][h] = j < J ? (int)(widx(g, st.nR, 0, jj % 3, F0 + l16) * 8) : 0x7FFFFFF0;
+      offL[m][h] = j < J ? woff8(g, st.nL, jj % 3, E0 + l16, oob) : oob;
+      offR[m][h] = j < J ? woff8(g, st.nR, jj % 3, F0 + l16, oob) : oob;
       const _Float16 cf = j < J ? (_Float16)st.coeff : (_Float16)0.f;
       cre[m][h] = __builtin_bit_cast(uint32_t, h2v{cf, cf});
       cim[m][h] = __builtin_bit_cast(uint32_t, h2v{-cf, cf});
