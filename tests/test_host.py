@@ -123,3 +123,56 @@ def test_synthetic_inputs_are_valid():
         assert V.dtype == np.complex64
         G = np.einsum("tek,tfk->tef", V.reshape(2, 6, -1).conj().astype(complex), V.reshape(2, 6, -1).astype(complex))
         assert np.abs(G - np.eye(6)).max() < 1e-6
+
+
+TILE_ORDER_SRC = r"""
+#include "ed_kernels.hpp"
+#include <cstdio>
+#include <set>
+#include <utility>
+// every tile of an nt x nt grid (or of its upper triangle) exactly once, in
+// strips of ed::TSTRIP e-tile rows walked f-tile by f-tile
+int main() {
+  for (int nt = 1; nt <= 40; ++nt)
+    for (int m = 0; m < 2; ++m) {
+      const int per = m ? nt * (nt + 1) / 2 : nt * nt;
+      std::set<std::pair<int, int>> seen;
+      int prev_strip = 0;
+      for (int k = 0; k < per; ++k) {
+        int te = -1, tf = -1;
+        ed::decode_tile(k, nt, nt, m, te, tf);
+        if (te < 0 || te >= nt || tf < 0 || tf >= nt || (m && te > tf)) return 1;
+        if (te / ed::TSTRIP < prev_strip) return 2;  // strips in order
+        prev_strip = te / ed::TSTRIP;
+        seen.insert({te, tf});
+      }
+      if ((int)seen.size() != per) return 3;
+    }
+  int te, tf;
+  ed::decode_tile(5, 13, 13, false, te, tf);  // strip 0: (0,0) (1,0) (2,0) (3,0) (0,1) (1,1)
+  if (te != 1 || tf != 1) return 4;
+  ed::decode_tile(3, 13, 13, true, te, tf);   // triangle strip 0: (0,0) (0,1) (1,1) (0,2)
+  if (te != 0 || tf != 2) return 5;
+  std::printf("ok\n");
+  return 0;
+}
+"""
+
+
+def test_contraction_tile_order_is_a_bijection(tmp_path):
+    """decode_tile (the contraction's work-item -> tile map) visits every tile of
+    the full grid and of the self-partnered upper triangle exactly once."""
+    import shutil
+    import subprocess
+
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "tile_order.cpp"
+    src.write_text(TILE_ORDER_SRC)
+    exe = tmp_path / "tile_order"
+    csrc = os.path.join(ROOT, "easydistillation_amd", "csrc")
+    subprocess.run([hipcc, "-w", "-O1", "-std=c++17", "--cuda-host-only", "-I", csrc, "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe)], check=True, capture_output=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0 and res.stdout.strip() == "ok", res.returncode
