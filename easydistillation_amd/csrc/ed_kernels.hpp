@@ -410,12 +410,16 @@ struct ContractArgs {
 
 // Tile k of one (timeslice, derivative, momentum block): all nte x ntf tiles,
 // or for a self-partnered derivative the nte (nte + 1) / 2 with te <= tf.
-// Tiles run in strips of TSTRIP e-tile rows: within a strip f-tile by f-tile,
-// and within an f-tile the strip's rows, so 32 consecutive items (one per CU
-// of an XCD, started together) read about 4 e slabs and 8 f slabs of the
-// workspace, which then hit in that XCD's L2.
+// Tiles run in strips of TS e-tile rows: within a strip f-tile by f-tile, and
+// within an f-tile the strip's rows, so 32 consecutive items (one per CU of
+// an XCD, started together) read about TS e slabs and 32 / TS f slabs of the
+// workspace, which then hit in that XCD's L2 (TS = 4 for the mixed kernels,
+// 2 for the FP64/FP32 kernel, measured).
 constexpr int TSTRIP = 4;
+constexpr int TSTRIP_FP = 2;
+template <int TS = TSTRIP>
 __host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror, int& te, int& tf) {
+  constexpr int TSTRIP = TS;
   int s0 = 0;
   if (mirror) {
     for (;;) {  // strip rows [s0, s0 + h), columns tf >= s0, te <= tf
@@ -450,6 +454,7 @@ __host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror
 // Work item w -> (timeslice tb, derivative di, momentum block pb, tile
 // (te, tf)).  Items of one timeslice run derivative by derivative
 // (DPlan::item0), each over npb blocks of its tiles (decode_tile).
+template <int TS = TSTRIP>
 __device__ __forceinline__ void decode_item(const ContractArgs& a, int w, int& tb, int& di, int& pb, int& te,
                                             int& tf) {
   tb = w / a.items_t;
@@ -460,7 +465,7 @@ __device__ __forceinline__ void decode_item(const ContractArgs& a, int w, int& t
   const bool mirror = a.dplan[di].mirror;
   const int per = mirror ? a.nte * (a.nte + 1) / 2 : a.nte * a.ntf;
   pb = r / per;
-  decode_tile(r - pb * per, a.nte, a.ntf, mirror, te, tf);
+  decode_tile<TS>(r - pb * per, a.nte, a.ntf, mirror, te, tf);
 }
 
 constexpr int SC = 16;      // sites per chunk (4 k-steps of the stage-2 MFMA)
@@ -496,7 +501,7 @@ __device__ __forceinline__ void contract_body(const c_t<T>* __restrict__ W, cons
   const int l16 = lane & 15, g4 = lane >> 4;
 
   int tb, di, pb, te, tf;
-  decode_item(a, w, tb, di, pb, te, tf);
+  decode_item<TSTRIP_FP>(a, w, tb, di, pb, te, tf);
   const DPlan dp = a.dplan[di];
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;

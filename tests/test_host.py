@@ -131,8 +131,9 @@ TILE_ORDER_SRC = r"""
 #include <set>
 #include <utility>
 // every tile of an nt x nt grid (or of its upper triangle) exactly once, in
-// strips of ed::TSTRIP e-tile rows walked f-tile by f-tile
-int main() {
+// strips of TS e-tile rows walked f-tile by f-tile
+template <int TS>
+int check() {
   for (int nt = 1; nt <= 40; ++nt)
     for (int m = 0; m < 2; ++m) {
       const int per = m ? nt * (nt + 1) / 2 : nt * nt;
@@ -140,18 +141,23 @@ int main() {
       int prev_strip = 0;
       for (int k = 0; k < per; ++k) {
         int te = -1, tf = -1;
-        ed::decode_tile(k, nt, nt, m, te, tf);
+        ed::decode_tile<TS>(k, nt, nt, m, te, tf);
         if (te < 0 || te >= nt || tf < 0 || tf >= nt || (m && te > tf)) return 1;
-        if (te / ed::TSTRIP < prev_strip) return 2;  // strips in order
-        prev_strip = te / ed::TSTRIP;
+        if (te / TS < prev_strip) return 2;  // strips in order
+        prev_strip = te / TS;
         seen.insert({te, tf});
       }
       if ((int)seen.size() != per) return 3;
     }
+  return 0;
+}
+int main() {
+  if (int r = check<ed::TSTRIP>()) return r;
+  if (int r = check<ed::TSTRIP_FP>()) return 10 + r;
   int te, tf;
-  ed::decode_tile(5, 13, 13, false, te, tf);  // strip 0: (0,0) (1,0) (2,0) (3,0) (0,1) (1,1)
+  ed::decode_tile<4>(5, 13, 13, false, te, tf);  // strip 0: (0,0) (1,0) (2,0) (3,0) (0,1) (1,1)
   if (te != 1 || tf != 1) return 4;
-  ed::decode_tile(3, 13, 13, true, te, tf);   // triangle strip 0: (0,0) (0,1) (1,1) (0,2)
+  ed::decode_tile<4>(3, 13, 13, true, te, tf);   // triangle strip 0: (0,0) (0,1) (1,1) (0,2)
   if (te != 0 || tf != 2) return 5;
   std::printf("ok\n");
   return 0;
