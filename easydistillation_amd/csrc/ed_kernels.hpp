@@ -1277,7 +1277,8 @@ __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, 
 template <int NKS>
 __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict__ W, const half8* __restrict__ ph16,
                                                        double2* __restrict__ out, const ContractArgs& a, const int w,
-                                                       float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE]) {
+                                                       float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE],
+                                                       half8 (*PHs)[4][64]) {
   typedef float4v acc_t;
   typedef float v2f __attribute__((ext_vector_type(2)));
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
@@ -1305,15 +1306,16 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
   const int cw = wave - NPROD;
   const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4;
   const __amdgpu_buffer_rsrc_t pr_ = __builtin_amdgcn_make_buffer_rsrc((void*)PH, (short)0, nchunk * 4 * 16 * 4 * 16, 0x00020000);
+  // The chunk's phase operand, 4 parts (Re hi, Re lo, Im hi, Im lo) of 1 KB,
+  // crosses the load path once per workgroup: consumer cw loads part cw two
+  // chunks ahead, stores it to LDS one chunk ahead (double buffer PHs), and
+  // every consumer reads all four parts from LDS.  Lane (l16, g4) holds
+  // momentum l16, sites 8 g4 .. 8 g4 + 7 of a part.
   const int phoff = (l16 * 4 + g4) * 16;
-  half8 prh, prl, pih, pil;
-  auto load_phase = [&](int chunk) {
-    const int co = chunk * 4 * 16 * 4 * 16;  // uniform
-    prh = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 0 * 1024, 0));
-    prl = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 1 * 1024, 0));
-    pih = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 2 * 1024, 0));
-    pil = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 3 * 1024, 0));
+  auto load_part = [&](int chunk) {
+    return __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, chunk * 4096 + cw * 1024, 0));
   };
+  half8 prh, prl, pih, pil;
   double D1[NBC][4], D2[NBC][4], D3[NBC][4], D4[NBC][4];
   float cr[NBC], ci[NBC];
 #pragma unroll
@@ -1366,8 +1368,10 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
     }
   };
 
-  load_phase(0);
-  __syncthreads();
+  half8 nxt = load_part(0);
+  PHs[0][cw][lane] = nxt;
+  if (nchunk > 1) nxt = load_part(1);
+  __syncthreads();  // chunk 0: Q and phase in LDS
   {
     const int n0 = min(SC2, g.V3);
 #pragma unroll
@@ -1383,7 +1387,11 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
     }
   }
   for (int chunk = 0; chunk < nchunk; ++chunk) {
-    const int c0 = chunk * SC2, buf = chunk & 1;
+    const int buf = chunk & 1;
+    prh = PHs[buf][0][lane];
+    prl = PHs[buf][1][lane];
+    pih = PHs[buf][2][lane];
+    pil = PHs[buf][3][lane];
     float tmax = 0.f;
 #pragma unroll
     for (int v = 0; v < NPROD; ++v) tmax = fmaxf(tmax, wmax[buf][v]);
@@ -1398,7 +1406,9 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
       stage2(b, qb, sq, ex);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (chunk + 1 < nchunk) load_phase(chunk + 1);
+    // buffer (chunk + 1) & 1 was last read in the previous chunk
+    if (chunk + 1 < nchunk) PHs[(chunk + 1) & 1][cw][lane] = nxt;
+    if (chunk + 2 < nchunk) nxt = load_part(chunk + 2);
     __syncthreads();
   }
   __syncthreads();  // the producers' zero-momentum partial sums are in LDS
@@ -1518,9 +1528,10 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_ws_kernel(const 
                                                                            ContractArgs a) {
   __shared__ float2 Qs[2][SC2][QSITE];
   __shared__ float wmax[2][NWAVE];
+  __shared__ half8 PHs[2][4][64];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
-  contract_mixed_ws_body<NKS>(W, ph16, out, a, w, Qs, wmax);
+  contract_mixed_ws_body<NKS>(W, ph16, out, a, w, Qs, wmax, PHs);
 }
 
 template <typename T, int NKS>
@@ -1545,6 +1556,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(con
                                                                              int nprob, int total, int order) {
   __shared__ float2 Qs[2][SC2][QSITE];
   __shared__ float wmax[2][NWAVE];
+  __shared__ half8 PHs[2][4][64];
   const int per = (total + 7) >> 3;
   const int nj = gridDim.x >> 3;
   for (int k = blockIdx.x >> 3; k < per; k += nj) {
@@ -1555,7 +1567,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(con
     __syncthreads();
     if constexpr (NKS <= 3)
       contract_mixed_ws_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
-                                  item - probs[i].work0, Qs, wmax);
+                                  item - probs[i].work0, Qs, wmax, PHs);
     else
       contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
                                item - probs[i].work0, (float2(*)[SC2][16][16])(void*)Qs, wmax);
