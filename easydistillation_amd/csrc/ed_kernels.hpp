@@ -267,26 +267,54 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
       lb[k] = ub[k];
     }
     const int child = child0 + 3 * pl + mu;
-    for (int e = lane; e < g.Ne; e += 64) {
-      double2 af[3], ab[3];
+    // (U_mu(x) af - U_mu(x - mu)^dag ab)_a for one eigenvector
+    auto hop = [&](const double2* af, const double2* ab, int a) {
+      double re = 0, im = 0;
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        af[b] = wget(Wt[widx(g, parent, sf[mu], b, e)], g.wsh);
-        ab[b] = wget(Wt[widx(g, parent, sb[mu], b, e)], g.wsh);
+        const double2 u = lf[a * 3 + b];
+        re += u.x * af[b].x - u.y * af[b].y;
+        im += u.x * af[b].y + u.y * af[b].x;
+        const double2 v = lb[b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
+        re -= v.x * ab[b].x + v.y * ab[b].y;
+        im -= v.x * ab[b].y - v.y * ab[b].x;
       }
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        double re = 0, im = 0;
+      return wput<c_t<T>>(re, im, g.wsh);
+    };
+    if constexpr (sizeof(c_t<T>) == 8) {
+      // 8-byte elements: each lane takes the eigenvector pair (2 l, 2 l + 1),
+      // one 16-byte load per colour and neighbour (NeP is even; the padded
+      // partner of an odd Ne is zero and stays zero)
+      typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+      typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+      for (int e = 2 * lane; e < g.Ne; e += 128) {
+        double2 af[2][3], ab[2][3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-          const double2 u = lf[a * 3 + b];
-          re += u.x * af[b].x - u.y * af[b].y;
-          im += u.x * af[b].y + u.y * af[b].x;
-          const double2 v = lb[b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
-          re -= v.x * ab[b].x + v.y * ab[b].y;
-          im -= v.x * ab[b].y - v.y * ab[b].x;
+          const u4 pf = *(const u4*)&Wt[widx(g, parent, sf[mu], b, e)];
+          const u4 pb = *(const u4*)&Wt[widx(g, parent, sb[mu], b, e)];
+          af[0][b] = wget(__builtin_bit_cast(c_t<T>, u2{pf.x, pf.y}), g.wsh);
+          af[1][b] = wget(__builtin_bit_cast(c_t<T>, u2{pf.z, pf.w}), g.wsh);
+          ab[0][b] = wget(__builtin_bit_cast(c_t<T>, u2{pb.x, pb.y}), g.wsh);
+          ab[1][b] = wget(__builtin_bit_cast(c_t<T>, u2{pb.z, pb.w}), g.wsh);
         }
-        Wo[widx(g, child, s, a, e)] = wput<c_t<T>>(re, im, g.wsh);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const u2 h0 = __builtin_bit_cast(u2, hop(af[0], ab[0], a));
+          const u2 h1 = __builtin_bit_cast(u2, hop(af[1], ab[1], a));
+          *(u4*)&Wo[widx(g, child, s, a, e)] = u4{h0.x, h0.y, h1.x, h1.y};
+        }
+      }
+    } else {
+      for (int e = lane; e < g.Ne; e += 64) {
+        double2 af[3], ab[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          af[b] = wget(Wt[widx(g, parent, sf[mu], b, e)], g.wsh);
+          ab[b] = wget(Wt[widx(g, parent, sb[mu], b, e)], g.wsh);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Wo[widx(g, child, s, a, e)] = hop(af, ab, a);
       }
     }
   }
