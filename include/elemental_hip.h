@@ -124,9 +124,10 @@ int ed_synchronize(ed_context* ctx);
  *   ED_OPT_WS_MB          workspace budget in MiB for the automatic batch (default 36864).
  *   ED_OPT_PROFILE        1: record HIP events around every kernel launch.
  *   ED_OPT_MIXED_STAGE2   mixed form's phase contraction: 1 FP16 MFMA on a three-term
- *                         hi/lo split, 0 FP32 MFMA, -1 (default) FP16 when V3 >= 4096
- *                         (small lattices keep FP32, whose error on blocks that are small
- *                         against their terms is ~2.5x lower).
+ *                         hi/lo split, 0 FP32 MFMA, -1 (default) FP16 when V3 >= 4096; with
+ *                         the default, lattices with V3 < 4096 run the FP64 kernels (there
+ *                         the FP32 stage-1 sum reaches ~1.2e-6 on (d, p) blocks that are
+ *                         small against their derivative's largest block).
  *   ED_OPT_EXACT_V        1: the next ed_set_eigvecs also keeps V in complex128 and the FP64
  *                         form and ed_backward use it unrounded (for derivative checks by
  *                         finite differences); 0 (default): V is rounded to complex64 as

@@ -46,11 +46,13 @@ def blockwise(E, Eref):
     return float((d / np.maximum(den, 1e-300)).max())
 
 
-def run(latt, Ne, num_nabla, moms, U, V, mode="fp64", hermitian=1, tbatch=0, dilution=None):
+def run(latt, Ne, num_nabla, moms, U, V, mode="fp64", hermitian=1, tbatch=0, dilution=None, stage2=None):
     g = ed.ElementalGenerator(
         latt, None, None, num_nabla, moms, dilution=dilution, is_blending=dilution is not None, Ne=Ne, mode=mode
     )
     g.set_option(_native.ED_OPT_HERMITIAN, hermitian)
+    if stage2 is not None:
+        g.set_option(_native.ED_OPT_MIXED_STAGE2, stage2)
     if tbatch:
         g.set_option(_native.ED_OPT_TBATCH, tbatch)
     g.load_arrays(U, V)
@@ -111,6 +113,12 @@ def test_uneven_shapes_against_oracle(case):
     errm = blockwise(Em, Eref)
     say(f"  mixed err={errm:.3e}")
     assert errm <= MIXED_TOL
+    # below V3 = 4096 the mixed form runs the FP64 kernels by default; the
+    # FP32 mixed kernel stays reachable and is checked on the same shapes
+    Em32 = run(latt, Ne, nn, moms, U, V, "mixed", stage2=0)
+    errm32 = blockwise(Em32, Eref)
+    say(f"  mixed FP32 kernel err={errm32:.3e}")
+    assert errm32 <= MIXED_TOL
 
 
 def test_blending_against_oracle():
