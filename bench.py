@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # in the same run gives (2.00 x FETCH_SIZE, 1.00 x WRITE_SIZE); the summary and the commands are in
 # profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, warp-specialized).
 TRAFFIC_MEASURED = {
-    ("C", "mixed", 1): (134220592.8 * 2.0 + 3223028.7) * 1024,  # contract_mixed_ws_kernel<3>, 8 timeslices
-    ("C", "fp64", 1): (463718872.8 * 2.0 + 2752433.9) * 1024,  # contract_kernel<double, 3>, 8 timeslices
-    ("C", "fp64", 0): (463718872.8 * 2.0 + 2752433.9) * 1024,
+    ("C", "mixed", 1): (127743801.0 * 2.0 + 3239083.4) * 1024,  # contract_mixed_ws_kernel<3>, 8 timeslices
+    ("C", "fp64", 1): (434679524.2 * 2.0 + 2751779.8) * 1024,  # contract_kernel<double, 3>, 8 timeslices
+    ("C", "fp64", 0): (434679524.2 * 2.0 + 2751779.8) * 1024,
 }
 TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
 MIXED_KERNEL = {2: "contract_mixed16_kernel<2>", 1: "contract_mixed_ws_kernel<3>", 0: "contract_mixed_f16_kernel<3>"}
