@@ -1126,11 +1126,12 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
 // FP32 MFMA split sum, 8 sites each per 32-site chunk -- and publish Q and
 // its tile maximum in LDS; waves 4..7 (consumers) run stage 2 -- scaling,
 // FP16 hi/lo split and FP16 MFMA phase contraction, 4 eigenvector rows each.
-// Wave w and wave w + 4 share a SIMD, so one wave's conversion work overlaps
-// the other's matrix instructions.  The roles keep separate loops with the
-// same number of barriers (1 + nchunk), so neither role's registers are live
-// in the other.  Numerics, LDS layout and epilogue are those of
-// contract_mixed_body.
+// Wave w and wave w + 4 share a SIMD (a SIMD issues no vector ALU work while
+// it runs FP32 16x16x4 MFMAs, so the two roles interleave rather than
+// overlap there); what bounds the kernel is the producers' workspace loads,
+// the CU's load path.  The roles keep separate loops with the same number of
+// barriers (nchunk + 2), so neither role's registers are live in the other.
+// Numerics, LDS layout and epilogue are those of contract_mixed_body.
 // --------------------------------------------------------------------------
 constexpr int NPROD = 4;                   // producer waves
 // LDS row-major Q tile per site, padded by 16 bytes: the consumer's four lane
