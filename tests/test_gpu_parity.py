@@ -304,3 +304,20 @@ def test_load_from_ildg_and_qdp_files(tmp_path):
     err = blockwise(E, O.elemental(V, U, latt, nn, moms))
     say(f"ILDG + QDP containers: err={err:.3e}")
     assert err <= FP64_TOL
+
+
+def test_mixed_below_4096_sites_runs_fp64_kernels():
+    """Below V3 = 4096 the mixed form's default is the FP64 kernels (bit-identical to mode
+    "fp64"); an explicit ED_OPT_MIXED_STAGE2 selects the mixed kernels again."""
+    latt, Ne, nn = [6, 5, 7, 2], 12, 2
+    moms = S.momenta_by_norm(2)
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=5)
+    say("mixed routing below V3 = 4096")
+    E64 = run(latt, Ne, nn, moms, U, V, "fp64")
+    Em = run(latt, Ne, nn, moms, U, V, "mixed")
+    assert np.array_equal(E64, Em)
+    Em32 = run(latt, Ne, nn, moms, U, V, "mixed", stage2=0)
+    assert not np.array_equal(E64, Em32)
+    # the forced FP32 kernel on a lattice this small sits near the mixed bound
+    # (the reason for the routing); here it only has to be the mixed kernel
+    assert blockwise(Em32, E64) <= 1e-5
