@@ -447,17 +447,16 @@ constexpr int TSTRIP = 4;
 constexpr int TSTRIP_FP = 2;
 template <int TS = TSTRIP>
 __host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror, int& te, int& tf) {
-  constexpr int TSTRIP = TS;
   int s0 = 0;
   if (mirror) {
     for (;;) {  // strip rows [s0, s0 + h), columns tf >= s0, te <= tf
-      const int h = min(TSTRIP, nte - s0);
+      const int h = min(TS, nte - s0);
       const int cnt = h * (nte - s0) - h * (h - 1) / 2;
       if (k < cnt) break;
       k -= cnt;
-      s0 += TSTRIP;
+      s0 += TS;
     }
-    const int h = min(TSTRIP, nte - s0);
+    const int h = min(TS, nte - s0);
     tf = s0;
     for (;;) {
       const int c = min(h, tf - s0 + 1);
@@ -468,12 +467,12 @@ __host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror
     te = s0 + k;
   } else {
     for (;;) {
-      const int cnt = min(TSTRIP, nte - s0) * ntf;
+      const int cnt = min(TS, nte - s0) * ntf;
       if (k < cnt) break;
       k -= cnt;
-      s0 += TSTRIP;
+      s0 += TS;
     }
-    const int h = min(TSTRIP, nte - s0);
+    const int h = min(TS, nte - s0);
     tf = k / h;
     te = s0 + (k - tf * h);
   }
