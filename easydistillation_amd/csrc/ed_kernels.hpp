@@ -152,76 +152,9 @@ __global__ __launch_bounds__(256) void stage_v_kernel(const VT* __restrict__ V, 
 }
 
 // --------------------------------------------------------------------------
-// Covariant hop (elemental.py:282-287), periodic:
-//   out(x) = U_mu(x) in(x + mu) - U_mu(x - mu)^dag in(x - mu)
-// One launch computes every child of one derivative level: blockIdx.y
-// enumerates (parent, mu); parent = parent0 + y / 3, child = child0 + y.
-// A wave owns one site (so both links are wave-uniform loads) and its lanes
-// walk the eigenvectors, reading and writing contiguous segments.
-// Arithmetic is FP64 for both forms; the complex64 form rounds once, when
-// the result is stored.
-// grid (ceil(V3 / 4), 3 * nparent, TB), block 256 (4 sites)
-// --------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void hop_kernel(c_t<T>* __restrict__ W, const double2* __restrict__ U, Geom g,
-                                                   int tb0, int parent0, int child0) {
-  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (s >= g.V3) return;
-  const int y = blockIdx.y;
-  const int parent = parent0 + y / 3, mu = y % 3, child = child0 + y;
-  const int tb = blockIdx.z;
-  const int x = s % g.Lx, yy = (s / g.Lx) % g.Ly, z = s / (g.Lx * g.Ly);
-  int sf, sb;
-  if (mu == 0) {
-    sf = s - x + (x + 1 == g.Lx ? 0 : x + 1);
-    sb = s - x + (x == 0 ? g.Lx - 1 : x - 1);
-  } else if (mu == 1) {
-    const int r = s - yy * g.Lx;
-    sf = r + (yy + 1 == g.Ly ? 0 : yy + 1) * g.Lx;
-    sb = r + (yy == 0 ? g.Ly - 1 : yy - 1) * g.Lx;
-  } else {
-    const int r = s - z * g.Lx * g.Ly;
-    sf = r + (z + 1 == g.Lz ? 0 : z + 1) * g.Lx * g.Ly;
-    sb = r + (z == 0 ? g.Lz - 1 : z - 1) * g.Lx * g.Ly;
-  }
-  const double2* Um = U + ((size_t)mu * g.Lt + (tb0 + tb)) * (size_t)g.V3 * 9;
-  double2 uf[9], ub[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    uf[k] = Um[(size_t)s * 9 + k];
-    ub[k] = Um[(size_t)sb * 9 + k];
-  }
-  const c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;
-  c_t<T>* Wo = const_cast<c_t<T>*>(Wt);
-  for (int e = lane; e < g.Ne; e += 64) {
-    double2 af[3], ab[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const c_t<T> vf = Wt[widx(g, parent, sf, b, e)], vb = Wt[widx(g, parent, sb, b, e)];
-      af[b] = make_double2(vf.x, vf.y);
-      ab[b] = make_double2(vb.x, vb.y);
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double re = 0, im = 0;
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const double2 u = uf[a * 3 + b];
-        re += u.x * af[b].x - u.y * af[b].y;
-        im += u.x * af[b].y + u.y * af[b].x;
-        const double2 v = ub[b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
-        re -= v.x * ab[b].x + v.y * ab[b].y;
-        im -= v.x * ab[b].y - v.y * ab[b].x;
-      }
-      Wo[widx(g, child, s, a, e)] = mk<T>((T)re, (T)im);
-    }
-  }
-}
-
-// --------------------------------------------------------------------------
-// Covariant hops of one derivative level, all three directions of a parent in
-// one pass: child(parent, mu)(x) = U_mu(x) in(x + mu) - U_mu(x - mu)^dag in(x - mu)
+// Covariant hops (elemental.py:282-287, periodic) of one derivative level, all
+// three directions of a parent in one pass:
+//   child(parent, mu)(x) = U_mu(x) in(x + mu) - U_mu(x - mu)^dag in(x - mu)
 // for mu = x, y, z.  The six neighbour reads of a parent value by the three
 // children of one site share the caches, so the parent is fetched from HBM
 // about once per level instead of twice per child.  The site index is made
