@@ -1047,6 +1047,7 @@ void batch_enqueue(ed_batch* b) {
   do {                                                                                                  \
     if (k == 0) ed::contract_group_kernel<double, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);   \
     else if (k == 1) ed::contract_group_kernel<float, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord); \
+    else if (b->grid_per_cu <= 0) ed::contract_mixed_item_kernel<KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord); \
     else ed::contract_mixed_group_kernel<KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);            \
   } while (0)
     if (ks == 1) ED_GROUP(1);

@@ -1535,4 +1535,25 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(con
   }
 }
 
+// One workgroup per work item of the problem table (the batch's launch shape
+// "one workgroup per item"): no persistent loop, so no state lives across
+// items.  Grid = 8 ceil(total / 8); item mapping as contract_mixed_group_kernel.
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_item_kernel(const GroupProblem* __restrict__ probs,
+                                                                            int nprob, int total, int order) {
+  __shared__ float2 Qs[2][SC2][QSITE];
+  __shared__ float wmax[2][NWAVE];
+  __shared__ half8 PHs[2][4][64];
+  const int item = __builtin_amdgcn_readfirstlane(group_item(blockIdx.x >> 3, (total + 7) >> 3, order));
+  if (item >= total) return;
+  const int i = __builtin_amdgcn_readfirstlane(group_find(probs, nprob, item));  // uniform: scalar loads
+  const ContractArgs a = probs[i].a;
+  if constexpr (NKS <= 3)
+    contract_mixed_ws_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                                item - probs[i].work0, Qs, wmax, PHs);
+  else
+    contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                             item - probs[i].work0, (float2(*)[SC2][16][16])(void*)Qs, wmax);
+}
+
 }  // namespace ed
