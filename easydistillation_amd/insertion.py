@@ -1,7 +1,10 @@
 """Derivative index map and momentum phases of the elemental path.
 
 ``derivative`` and ``output`` follow lattice/insertion/derivative.py:1-33;
-``MomentumPhase.get`` follows lattice/insertion/phase.py:11-13,41-46.
+``MomentumPhase.get`` follows lattice/insertion/phase.py:11-13,41-46;
+``momentum_dict`` / ``mom_dict_to_list`` give the momentum orders of
+lattice/insertion/mom_dict.py:1-175 (the order in which downstream readers
+of an elemental file, insertion/__init__.py:195-198, index momenta).
 These are host-side helpers: the HIP library rebuilds the same index map
 and an FP64 phase table itself.
 """
@@ -85,3 +88,35 @@ class MomentumPhase:
             frac = z[:, None, None] + y[None, :, None] + x[None, None, :]
             self.cache[key] = np.exp(2j * np.pi * frac)
         return self.cache[key]
+
+
+def momentum_dict(mom: int = 9) -> Dict[int, str]:
+    """Momentum table {index: "nx ny nz"} in the reference's order (mom_dict.py:1-175).
+
+    mom = 1: the zero momentum.  mom = 3: the 27 n with components in
+    {-1, 0, 1}, by |n|^2, then by which components are nonzero (mask
+    4 [nx != 0] + 2 [ny != 0] + [nz != 0], ascending), then + before - in
+    x, y, z order.  mom = 9: the 123 n with |n|^2 <= 9, by nz, then ny, then
+    nx, ascending.
+    """
+    import itertools
+
+    if mom == 1:
+        moms = [(0, 0, 0)]
+    elif mom == 3:
+        mask = lambda n: 4 * (n[0] != 0) + 2 * (n[1] != 0) + (n[2] != 0)  # noqa: E731
+        moms = sorted(itertools.product((-1, 0, 1), repeat=3),
+                      key=lambda n: (sum(c * c for c in n), mask(n), tuple(-c for c in n)))
+    elif mom == 9:
+        moms = sorted((n for n in itertools.product(range(-3, 4), repeat=3) if sum(c * c for c in n) <= 9),
+                      key=lambda n: (n[2], n[1], n[0]))
+    else:
+        raise ValueError(f"Unknown mom max = {mom}")
+    return {i: f"{n[0]} {n[1]} {n[2]}" for i, n in enumerate(moms)}
+
+
+def mom_dict_to_list(mom: int = 9) -> List[Tuple[int, int, int]]:
+    """The momenta of ``momentum_dict(mom)`` as integer triples, in order
+    (mom_dict.py:162-175): the momentum_list to pass to ElementalGenerator for
+    files read with that table."""
+    return [tuple(int(p) for p in v.split(" ")) for v in momentum_dict(mom).values()]

@@ -182,3 +182,25 @@ def test_contraction_tile_order_is_a_bijection(tmp_path):
                     str(src), "-o", str(exe)], check=True, capture_output=True)
     res = subprocess.run([str(exe)], capture_output=True, text=True)
     assert res.returncode == 0 and res.stdout.strip() == "ok", res.returncode
+
+
+def test_momentum_dict_orders_match_the_reference_tables():
+    """momentum_dict reproduces the reference's momentum tables
+    (lattice/insertion/mom_dict.py:1-175, extracted as data by
+    tests/golden/make_mom_tables.py) and each table holds every momentum of
+    its set exactly once."""
+    import itertools
+    import json
+
+    from easydistillation_amd.insertion import mom_dict_to_list, momentum_dict
+
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "mom_dict_tables.json")))["tables"]
+    for mom in (1, 3, 9):
+        ours = mom_dict_to_list(mom)
+        assert [list(n) for n in ours] == ref[f"mom{mom}"], mom
+        assert momentum_dict(mom) == {i: " ".join(str(c) for c in n) for i, n in enumerate(ours)}
+    assert set(mom_dict_to_list(3)) == set(itertools.product((-1, 0, 1), repeat=3))
+    ball = {n for n in itertools.product(range(-3, 4), repeat=3) if sum(c * c for c in n) <= 9}
+    assert len(mom_dict_to_list(9)) == len(ball) == 123 and set(mom_dict_to_list(9)) == ball
+    with pytest.raises(ValueError):
+        momentum_dict(5)
