@@ -21,7 +21,7 @@ norm <= tol * max(|lambda|, 1).  The eigenvectors stay on the device
 
 from __future__ import annotations
 
-from typing import List, Sequence
+from typing import List
 
 import numpy as np
 import torch

@@ -19,7 +19,7 @@ takes links and eigenvectors directly (numpy or device tensors), and
 
 from __future__ import annotations
 
-from typing import List, Sequence, Tuple
+from typing import List, Tuple
 
 import numpy as np
 

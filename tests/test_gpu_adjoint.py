@@ -15,7 +15,7 @@ import torch
 
 from easydistillation_amd import _native
 from easydistillation_amd import synthetic as S
-from easydistillation_amd.autograd import ElementalEngine, elemental, profiled_loss
+from easydistillation_amd.autograd import ElementalEngine, profiled_loss
 from oracle import elemental_oracle as O
 from oracle import elemental_torch as T
 

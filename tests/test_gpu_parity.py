@@ -5,7 +5,6 @@ max|dE| / max|E| of every (d, p) block (elementwise relative error is
 ill-conditioned on near-zero elementals).
 """
 
-import itertools
 import sys
 
 import numpy as np

@@ -1,6 +1,5 @@
 """Host logic and C-ABI surface, on the CPU (no GPU compute calls)."""
 
-import ctypes
 import os
 import re
 
