@@ -15,6 +15,7 @@ from .preset import (
     EigenvectorHostmem,
     EigenvectorNpy,
     EigenvectorTimeSlice,
+    ElementalBinary,
     ElementalNpy,
     GaugeFieldHostmem,
     GaugeFieldIldg,
@@ -40,6 +41,7 @@ __all__ = [
     "EigenvectorHostmem",
     "EigenvectorNpy",
     "EigenvectorTimeSlice",
+    "ElementalBinary",
     "ElementalNpy",
     "GaugeFieldIldg",
     "GaugeFieldHostmem",

@@ -15,6 +15,10 @@
   (number of records, then per record an int tuple key and a position).
   Each record holds one array of the element shape (big-endian).
 
+* Headerless binary arrays (lattice/filedata/binary.py:15-60, used through
+  ``ElementalBinary``, lattice/preset.py:129-137): the array in C order in
+  the file's dtype (``"<c16"`` for elementals), read back in that dtype.
+
 Readers memory-map the file and return native complex arrays for the
 requested key, as the reference's ``FileData.__getitem__``.  ``write_ildg``
 and ``write_qdp_lazy`` produce the same layouts (for tests and for moving
@@ -249,3 +253,64 @@ def write_qdp_lazy(path: str, records: Dict[Tuple[int, ...], np.ndarray], latt_s
         f.write(struct.pack(">qq", 0, pos + len(body)))
         f.write(body)
         f.write(table)
+
+
+# ----------------------------------------------------------------------------
+# Headerless binary arrays
+# ----------------------------------------------------------------------------
+class BinaryFileData:
+    """One headerless array file: ``shape`` in C order, elements of ``dtype``.
+
+    ``data[key]`` maps the file and returns a copy of ``array[key]`` in the
+    file's dtype (lattice/filedata/binary.py:29-60); a file shorter than the
+    shape raises ``ValueError``.
+    """
+
+    def __init__(self, file: str, shape: Sequence[int], dtype: str):
+        self.file = file
+        self.shape = [int(v) for v in shape]
+        self.dtype = dtype
+        self.bytes = _elem_bytes(dtype)
+        self.stride = [_prod(self.shape[i:]) for i in range(1, len(self.shape))] + [1]
+        self.time_in_sec = 0.0
+        self.size_in_byte = 0
+
+    def get_count(self, key: Tuple[int, ...]) -> int:
+        return self.stride[len(key) - 1]
+
+    def get_offset(self, key: Tuple[int, ...]) -> int:
+        return sum(int(a) * b for a, b in zip(key, self.stride)) * self.bytes
+
+    def __getitem__(self, key):
+        import os
+        import time
+
+        t0 = time.time()
+        need = _prod(self.shape) * self.bytes
+        have = os.path.getsize(self.file)
+        if have < need:
+            raise ValueError(f"{self.file}: {have} bytes, the shape {self.shape} of {self.dtype} needs {need}")
+        out = _mapped(self.file, 0, self.shape, self.dtype, key, np.dtype(self.dtype))
+        self.time_in_sec += time.time() - t0
+        self.size_in_byte += out.nbytes
+        return out
+
+
+class BinaryFile:
+    """Keeps the last opened file (lattice/filedata/binary.py:63-72)."""
+
+    def __init__(self):
+        self.file = None
+        self.data = None
+
+    def get_file_data(self, name: str, shape: Sequence[int], dtype: str) -> BinaryFileData:
+        if self.file != name:
+            self.data = BinaryFileData(name, shape, dtype)
+            self.file = name
+        return self.data
+
+
+def write_binary(path: str, array: np.ndarray, dtype: str = "<c16") -> None:
+    """Write ``array`` in C order as ``dtype``, no header."""
+    with open(path, "wb") as f:
+        f.write(np.ascontiguousarray(np.asarray(array).astype(dtype)).tobytes())

@@ -10,9 +10,10 @@ The classes here provide that contract for numpy ``.npy`` files (read through
 a memory map, so only the slices asked for are read), for arrays already in
 memory, for ILDG gauge configurations (``GaugeFieldIldg``,
 lattice/preset.py:151-160) and for QDP LazyDiskMapObj eigenvector files
-(``EigenvectorTimeSlice``, lattice/preset.py:55-64).  ``ElementalNpy`` reads
-elementals in the reference's file layout [D, P, Lt, Ne, Ne]
-(lattice/preset.py:173-181) and ``save_elemental`` writes it.
+(``EigenvectorTimeSlice``, lattice/preset.py:55-64).  ``ElementalNpy`` and
+``ElementalBinary`` read elementals in the reference's file layout
+[D, P, Lt, Ne, Ne] (lattice/preset.py:173-181, 129-137: .npy, and headerless
+little-endian complex128); ``save_elemental`` writes either.
 """
 
 from __future__ import annotations
@@ -21,7 +22,8 @@ from typing import List, Sequence
 
 import numpy as np
 
-from .filedata import IldgFile, IldgFileData, QDPLazyDiskMapObjFile, QDPLazyDiskMapObjFileData
+from .filedata import (BinaryFile, BinaryFileData, IldgFile, IldgFileData, QDPLazyDiskMapObjFile,
+                       QDPLazyDiskMapObjFileData, write_binary)
 
 
 class FileMetaData:
@@ -168,6 +170,29 @@ class ElementalNpy(_NpyFile):
         return self._open(key)
 
 
-def save_elemental(path: str, data_t_major: np.ndarray) -> None:
-    """Write [Lt, D, P, Ne, Ne] as the file layout [D, P, Lt, Ne, Ne] (tests/test_elemental.py:48)."""
-    np.save(path, np.ascontiguousarray(np.asarray(data_t_major).transpose(1, 2, 0, 3, 4)))
+class ElementalBinary(BinaryFile):
+    """Elementals as a headerless little-endian complex128 array [D, P, Lt, Ne, Ne]
+    in ``{prefix}{key}{suffix}`` (lattice/preset.py:129-137)."""
+
+    def __init__(self, prefix: str, suffix: str | None, shape: Sequence[int] = (40, 27, 128, 70, 70),
+                 totNe: int = 70):
+        BinaryFile.__init__(self)
+        self.prefix = prefix
+        self.suffix = ".stout.n20.f0.12.nev70.meson" if suffix is None else suffix
+        self.shape = list(shape)
+        self.Ne = totNe
+
+    def load(self, key: str) -> BinaryFileData:
+        return self.get_file_data(f"{self.prefix}{key}{self.suffix}", self.shape, "<c16")
+
+
+def save_elemental(path: str, data_t_major: np.ndarray, fmt: str = "npy") -> None:
+    """Write [Lt, D, P, Ne, Ne] as the file layout [D, P, Lt, Ne, Ne] (tests/test_elemental.py:48):
+    ``fmt`` "npy" (``ElementalNpy``) or "binary" (``ElementalBinary``, complex128, no header)."""
+    data = np.asarray(data_t_major).transpose(1, 2, 0, 3, 4)
+    if fmt == "npy":
+        np.save(path, np.ascontiguousarray(data))
+    elif fmt == "binary":
+        write_binary(str(path), data, "<c16")
+    else:
+        raise ValueError(f"fmt must be 'npy' or 'binary', not {fmt!r}")

@@ -1,7 +1,8 @@
 """ILDG/LIME and QDP LazyDiskMapObj readers (lattice/filedata/ildg.py:76-104,
 lattice/filedata/timeslice.py:102-129) against files assembled byte by byte
 from the published layouts in this test (not by the package's writers), and
-the writers against the readers.
+the writers against the readers; the headerless elemental binary layout
+(lattice/preset.py:129-137) byte by byte.
 """
 
 import struct
@@ -111,3 +112,33 @@ def test_qdp_writer_round_trip(tmp_path):
     data = EigenvectorTimeSlice(str(tmp_path) + "/", ".mod", [2, 3, 4, 3], 3).load("w")
     for k, a in recs.items():
         assert np.array_equal(data[k], a.astype(np.complex64))
+
+
+def test_elemental_binary_layout_and_reader(tmp_path):
+    # headerless little-endian complex128 in [D, P, Lt, Ne, Ne] C order
+    # (lattice/preset.py:129-137, lattice/filedata/binary.py:15-60)
+    from easydistillation_amd import ElementalBinary, save_elemental
+
+    rng = np.random.default_rng(7)
+    Lt, D, P, Ne = 3, 2, 4, 5
+    E = rng.normal(size=(Lt, D, P, Ne, Ne)) + 1j * rng.normal(size=(Lt, D, P, Ne, Ne))
+    path = tmp_path / "cfg.meson"
+    save_elemental(path, E, fmt="binary")
+    raw = path.read_bytes()
+    assert len(raw) == Lt * D * P * Ne * Ne * 16
+    for t, d, p, e, f in [(0, 0, 0, 0, 0), (2, 1, 3, 4, 4), (1, 0, 2, 3, 1), (2, 1, 0, 0, 3)]:
+        off = ((((d * P + p) * Lt + t) * Ne + e) * Ne + f) * 16
+        assert struct.unpack("<dd", raw[off:off + 16]) == (E[t, d, p, e, f].real, E[t, d, p, e, f].imag)
+    eb = ElementalBinary(f"{tmp_path}/", ".meson", [D, P, Lt, Ne, Ne], Ne)
+    data = eb.load("cfg")
+    assert eb.Ne == Ne and eb.load("cfg") is data
+    assert np.array_equal(data[:], E.transpose(1, 2, 0, 3, 4))
+    assert np.array_equal(data[1, 2], E[:, 1, 2])
+    assert data[1, 2].dtype == np.dtype("<c16")
+    # the reference's count and byte offset of a leading index tuple
+    assert data.get_count((1,)) == P * Lt * Ne * Ne and data.get_count((1, 2, 0)) == Ne * Ne
+    assert data.get_offset((1, 2)) == (1 * P + 2) * Lt * Ne * Ne * 16
+    with pytest.raises(ValueError):
+        ElementalBinary(f"{tmp_path}/", ".meson", [D, P, Lt + 1, Ne, Ne], Ne).load("cfg")[0]
+    with pytest.raises(ValueError):
+        save_elemental(path, E, fmt="hdf5")
