@@ -291,7 +291,7 @@ def main():
     if args.gpus != 1 or world != 1:
         print(
             "bench.py measures one GPU only: elemental generation shards over timeslices with no "
-            "communication, and this build runs on single-GPU boxes; run with --gpus 1",
+            "communication, so the one-GPU rate times the GPU count is the job's rate; run with --gpus 1",
             file=sys.stderr,
         )
         sys.exit(2)
