@@ -157,6 +157,7 @@ struct ed_context {
   // k-steps), 2 warp-specialized kernel with FP16 stage 1 (up to 2 k-groups),
   // 0 uniform waves
   int mixed_ws = 1;
+  int wide_ws = 0;  // 64-bit workspace addresses in the warp-specialized kernel below 2^31 bytes too
   float vmax = 0.f;        // largest |re|, |im| of the stored eigenvectors
   int wsh = 0;             // scale exponent of the FP16 stage-1 workspace
   int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
@@ -548,11 +549,20 @@ size_t per_t_ws_bytes(const ed_context* c) {
   return (size_t)c->slots() * c->NeP * 3 * c->V3P * sizeof(ed::c_t<T>);
 }
 
-// the mixed form takes the FP16-MFMA phase contraction (addressing one
-// timeslice's workspace with 32-bit buffer offsets) when allowed and chosen
+// one timeslice's FP32 workspace is addressable with 32-bit buffer offsets
+bool ws_fits32(const ed_context* c) { return per_t_ws_bytes<float>(c) < ((size_t)1 << 31); }
+
+// the warp-specialized FP32 stage-1 kernel, which also has a 64-bit-address
+// form for larger workspaces
+bool use_ws_kernel(const ed_context* c) { return c->mixed_ws != 0 && c->nks_max <= 3; }
+bool ws_wide(const ed_context* c) { return !ws_fits32(c) || c->wide_ws; }
+
+// the mixed form takes the FP16-MFMA phase contraction when chosen and its
+// kernel can address the workspace (every FP16 stage-2 kernel below 2^31
+// bytes per timeslice, the warp-specialized one at any size)
 bool use_f16_stage2(const ed_context* c) {
-  const bool fits32 = per_t_ws_bytes<float>(c) < ((size_t)1 << 31);
-  return fits32 && (c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096));
+  const bool chosen = c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096);
+  return chosen && (ws_fits32(c) || use_ws_kernel(c));
 }
 
 // The mixed form on lattices with V3 < 4096 and the default phase
@@ -570,7 +580,7 @@ bool mixed_runs_fp64(const ed_context* c) { return c->mixed_stage2 < 0 && c->V3 
 // small against their terms past 1e-6; there the FP32 stage 1 runs)
 int h16_groups(const ed_context* c) { return (3 * c->nsplit_max + 7) / 8; }
 bool use_h16(const ed_context* c) {
-  return use_f16_stage2(c) && c->mixed_ws == 2 && h16_groups(c) <= 2 && c->V3 >= 4096;
+  return use_f16_stage2(c) && ws_fits32(c) && c->mixed_ws == 2 && h16_groups(c) <= 2 && c->V3 >= 4096;
 }
 
 // Scale exponent of the FP16 stage-1 workspace: every stored component,
@@ -714,13 +724,21 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     if (sizeof(T) == 4 && use_f16_stage2(c)) {
       const float2* Wf = (const float2*)(const void*)W;
       c->launch(ED_KERNEL_CONTRACT, [&] {
-        if (c->mixed_ws && c->nks_max <= 3) {
+        if (use_ws_kernel(c)) {
+          const bool wide = ws_wide(c);
+          auto ws = [&](auto nks) {
+            constexpr int K = decltype(nks)::value;
+            if (wide)
+              ed::contract_mixed_ws_kernel<K, true><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+            else
+              ed::contract_mixed_ws_kernel<K, false><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+          };
           if (c->nks_max <= 1)
-            ed::contract_mixed_ws_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+            ws(std::integral_constant<int, 1>{});
           else if (c->nks_max <= 2)
-            ed::contract_mixed_ws_kernel<2><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+            ws(std::integral_constant<int, 2>{});
           else
-            ed::contract_mixed_ws_kernel<3><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
+            ws(std::integral_constant<int, 3>{});
         } else if (c->nks_max <= 1)
           ed::contract_mixed_f16_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wf, c->ph16.p, out, a);
         else if (c->nks_max <= 2)
@@ -949,9 +967,12 @@ int bguard(ed_batch* b, const std::function<void()>& body) {
   }
 }
 
+// the grouped kernels address each workspace with 32-bit buffer offsets: a
+// lattice whose timeslice workspace reaches 2^31 bytes runs the FP32 phase
+// contraction in a batch
 int batch_kind(const ed_context* c, int mode) {
   if (mode == ED_MODE_FP64 || mixed_runs_fp64(c)) return 0;
-  return use_f16_stage2(c) ? 2 : 1;
+  return use_f16_stage2(c) && ws_fits32(c) ? 2 : 1;
 }
 
 // Plans, workspaces (every timeslice at once) and the grouped problem tables;
@@ -1364,6 +1385,9 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
       case ED_OPT_MIXED_STAGE2:
         if (value < -1 || value > 1) fail(ED_ERR_ARG, "mixed stage-2 kind must be -1 (auto), 0 (FP32) or 1 (FP16 split)");
         ctx->mixed_stage2 = (int)value;
+        break;
+      case ED_OPT_WIDE_WS:
+        ctx->wide_ws = value ? 1 : 0;
         break;
       default:
         fail(ED_ERR_ARG, "unknown option %d", option);

@@ -787,7 +787,7 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
   const int E0 = te * 16, F0 = tf * 16;
   const Geom g = a.g;
   const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;  // bytes per site
-  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 (the FP16 phase contraction runs only then)
+  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 (the host runs this kernel only then)
   const float2* Wt = W + (size_t)tb * (slice_b / 8);
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
   const int J = __builtin_amdgcn_readfirstlane(3 * dp.nsplit);
@@ -1095,7 +1095,9 @@ __device__ __forceinline__ float wave_max_nonneg(float v) {
 // no branch inside the chunk loop, so the loads of the next chunk (issued as
 // soon as this chunk's copy of each operand is consumed; the last chunk
 // re-reads itself) stay in flight across the MFMAs and the barrier.
-template <int NK>
+// WIDE = false: 32-bit buffer offsets into one timeslice's workspace (below
+// 2^31 bytes); WIDE = true: 64-bit lane addresses, any workspace size.
+template <int NK, bool WIDE>
 __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, const ContractArgs& a, const DPlan& dp,
                                                   int tb, int E0, int F0, float2 (*Qs)[SC2][QSITE],
                                                   float (*wmax)[NWAVE]) {
@@ -1107,19 +1109,36 @@ __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, 
   const int nchunk = g.V3P / SC2;
   const acc_t zero = {0.f, 0.f, 0.f, 0.f};
   const uint32_t site_b = 3u * (uint32_t)g.NeP * 8u;     // bytes per site
-  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 (checked on the host)
+  const size_t slice_b = (size_t)g.nW * g.V3P * site_b;  // < 2^31 unless WIDE (chosen on the host)
   const float2* Wt = W + (size_t)tb * (slice_b / 8);
-  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, (int)slice_b, 0x00020000);
+  const __amdgpu_buffer_rsrc_t wr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)Wt, (short)0, WIDE ? 0 : (int)slice_b, 0x00020000);
   const int J = 3 * dp.nsplit;
-  // unused (split, colour) slots and padded rows read past the buffer's end
-  // (0, no memory request)
+  // 32-bit form: unused (split, colour) slots and padded rows read past the
+  // buffer's end (0, no memory request)
   const int oob = (int)slice_b;
   int offL[NK], offR[NK];
+  // wide form: every lane reads workspace values.  A padded row (e >= Ne)
+  // reads row Ne - 1, which lies in the same tile: its products are dropped
+  // at the store and leave the tile maximum unchanged.  An unused slot reads
+  // the first slot's operands with coefficient 0 (exact zero products, so the
+  // sums equal the 32-bit form's bit for bit).
+  const float2* pL[NK];
+  const float2* pR[NK];
+  const int site_e = 3 * g.NeP;  // elements per site
   float cf[NK];
 #pragma unroll
   for (int ks = 0; ks < NK; ++ks) {
     const int j = 4 * ks + g4;
-    if (j < J) {
+    if (WIDE) {
+      const bool use = j < J;
+      const SplitTerm st = a.splits[dp.first + (use ? j / 3 : 0)];
+      const int c = use ? j % 3 : 0;
+      pL[ks] = Wt + widx(g, st.nL, 0, c, min(E0 + l16, g.Ne - 1));
+      pR[ks] = Wt + widx(g, st.nR, 0, c, min(F0 + l16, g.Ne - 1));
+      cf[ks] = use ? st.coeff : 0.f;
+      offL[ks] = offR[ks] = 0;
+    } else if (j < J) {
       const SplitTerm st = a.splits[dp.first + j / 3];
       const int c = j % 3;
       offL[ks] = woff8(g, st.nL, c, E0 + l16, oob);
@@ -1132,6 +1151,17 @@ __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, 
   }
   v2f lv[NK][SPP], rv[NK][SPP];
   auto load = [&](int ks, bool left, int c0) {
+    if constexpr (WIDE) {
+      const size_t xo = (size_t)(c0 + SPP * wave) * site_e;  // uniform
+#pragma unroll
+      for (int i = 0; i < SPP; ++i) {
+        if (left)
+          lv[ks][i] = *(const v2f*)(pL[ks] + xo + i * site_e);
+        else
+          rv[ks][i] = *(const v2f*)(pR[ks] + xo + i * site_e);
+      }
+      return;
+    }
     const int xo = (c0 + SPP * wave) * (int)site_b;  // uniform
 #pragma unroll
     for (int i = 0; i < SPP; ++i) {
@@ -1235,7 +1265,7 @@ __device__ __forceinline__ void mixed_ws_producer(const float2* __restrict__ W, 
   __syncthreads();
 }
 
-template <int NKS>
+template <int NKS, bool WIDE = false>
 __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict__ W, const half8* __restrict__ ph16,
                                                        double2* __restrict__ out, const ContractArgs& a, const int w,
                                                        float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE],
@@ -1257,9 +1287,9 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
   if (wave < NPROD) {
     // producer: one branch-free instance per stage-1 k-step count
     const int nks = (3 * dp.nsplit + 3) >> 2;
-    if (NKS >= 3 && nks == 3) mixed_ws_producer<(NKS >= 3 ? 3 : 1)>(W, a, dp, tb, E0, F0, Qs, wmax);
-    else if (NKS >= 2 && nks == 2) mixed_ws_producer<(NKS >= 2 ? 2 : 1)>(W, a, dp, tb, E0, F0, Qs, wmax);
-    else mixed_ws_producer<1>(W, a, dp, tb, E0, F0, Qs, wmax);
+    if (NKS >= 3 && nks == 3) mixed_ws_producer<(NKS >= 3 ? 3 : 1), WIDE>(W, a, dp, tb, E0, F0, Qs, wmax);
+    else if (NKS >= 2 && nks == 2) mixed_ws_producer<(NKS >= 2 ? 2 : 1), WIDE>(W, a, dp, tb, E0, F0, Qs, wmax);
+    else mixed_ws_producer<1, WIDE>(W, a, dp, tb, E0, F0, Qs, wmax);
     return;
   }
 
@@ -1482,7 +1512,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
   contract_mixed_body<NKS>(W, ph16, out, a, w, Qs, wmax);
 }
 
-template <int NKS>
+template <int NKS, bool WIDE>
 __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_ws_kernel(const float2* __restrict__ W,
                                                                            const half8* __restrict__ ph16,
                                                                            double2* __restrict__ out,
@@ -1492,7 +1522,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_ws_kernel(const 
   __shared__ half8 PHs[2][4][64];
   const int w = xcd_remap(blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
-  contract_mixed_ws_body<NKS>(W, ph16, out, a, w, Qs, wmax, PHs);
+  contract_mixed_ws_body<NKS, WIDE>(W, ph16, out, a, w, Qs, wmax, PHs);
 }
 
 template <typename T, int NKS>

@@ -139,7 +139,13 @@ int ed_synchronize(ed_context* ctx);
  *                         largest eigenvector component; up to 16 (split, colour) terms and
  *                         V3 >= 4096, else kind 1 runs),
  *                         0 every wave runs both stages; derivatives with more (split,
- *                         colour) terms than the chosen kernel takes use the latter. */
+ *                         colour) terms than the chosen kernel takes use the latter.
+ *                         Kinds 0 and 2 address one timeslice's workspace with 32-bit
+ *                         offsets; from 2^31 bytes per timeslice (e.g. 32^3 with Nev 256 and
+ *                         two derivatives) kind 2 runs as kind 1, which has a 64-bit-address
+ *                         form, and kind 0 as the FP32 phase contraction.
+ *   ED_OPT_WIDE_WS        1: kind 1 uses its 64-bit-address form below 2^31 bytes too (same
+ *                         results bit for bit; for testing). */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
@@ -147,7 +153,8 @@ enum ed_option {
   ED_OPT_PROFILE = 4,
   ED_OPT_MIXED_STAGE2 = 5,
   ED_OPT_EXACT_V = 6,
-  ED_OPT_MIXED_WS = 7
+  ED_OPT_MIXED_WS = 7,
+  ED_OPT_WIDE_WS = 8
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 
