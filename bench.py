@@ -40,13 +40,23 @@ TRAFFIC_MEASURED = {
     ("C", "fp64", 0): (434679524.2 * 2.0 + 2751779.8) * 1024,
 }
 TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
-MIXED_KERNEL = {2: "contract_mixed16_kernel<2>", 1: "contract_mixed_ws_kernel<3>", 0: "contract_mixed_f16_kernel<3>"}
+# the library's ED_Q_MIXED_KERNEL answer -> the kernel the mixed form runs
+MIXED_KERNEL = {
+    0: "contract_kernel<double, 3>",
+    1: "contract_kernel<float, 3>",
+    2: "contract_mixed_f16_kernel<3>",
+    3: "contract_mixed_ws_kernel<3>",
+    4: "contract_mixed_ws_kernel<3> (64-bit workspace addresses)",
+    5: "contract_mixed16_kernel<2>",
+}
 
 WORKLOADS = {
     # name: (latt [Lx,Ly,Lz,Lt], Ne, num_nabla, max |n|^2, links)
     "C": ([32, 32, 32, 64], 200, 2, 4, "haar"),
     "B": ([16, 16, 16, 128], 100, 2, 4, "smooth"),
     "small": ([16, 16, 16, 8], 64, 2, 4, "haar"),
+    # 7.2 GB of FP32 workspace per timeslice: the mixed contraction's 64-bit-address form
+    "L": ([48, 48, 48, 8], 200, 2, 4, "smooth"),
 }
 # batched uneven form: three lattices in one graph launch, num_nabla 1 (4
 # derivative combinations), 19 momenta (|n|^2 <= 2), smooth links eps = 0.3
@@ -362,7 +372,8 @@ def main():
         s_ms, s_n = g.context.profile_read(_native.ED_KERNEL_STAGE)
         ex_flops_t = g.context.query(_native.ED_Q_CONTRACT_FLOPS_PER_T)
         f16_flops_t = 0
-        if form == "mixed" and Lx * Ly * Lz >= 4096:
+        mixed_kernel = g.context.query(_native.ED_Q_MIXED_KERNEL) if form == "mixed" else None
+        if mixed_kernel in (2, 3, 4, 5):
             # the mixed kernel runs stage 1 on FP32 MFMA and stage 2 on FP16 MFMA; its
             # roofline time is fp32_flops / FP32 peak + fp16_flops / FP16 peak, quoted
             # as FP32-equivalent flops
@@ -386,6 +397,7 @@ def main():
             "stencil_gbs": hop_bytes_t * Lt / (h_ms / args.steps * 1e-3) / 1e9 if h_ms > 0 else None,
             "stencil_ms_per_step": h_ms / args.steps,
             "stage_ms_per_step": s_ms / args.steps,
+            "mixed_kernel": mixed_kernel,
         }
         log(f"{form}: {json.dumps(results[form])}")
         outs[form] = out
@@ -435,7 +447,7 @@ def main():
             "frac": r["contract_tflops_executed"] / PEAK[head],
             "traffic": TRAFFIC_MEASURED.get(traffic_key),
             "traffic_source": TRAFFIC_SOURCE if traffic_key in TRAFFIC_MEASURED else None,
-            "kernel": MIXED_KERNEL[args.mixed_ws] if head == "mixed" else "contract_kernel<double, 3>",
+            "kernel": MIXED_KERNEL[r["mixed_kernel"]] if head == "mixed" else "contract_kernel<double, 3>",
             "flops_basis": (
                 "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
                 "MFMA flops are converted to FP32-equivalent at the FP32:FP16 peak ratio 157.3:2500"

@@ -65,6 +65,7 @@ ED_BOPT_GRID_PER_CU, ED_BOPT_ORDER, ED_BOPT_GRAPH = 1, 2, 3
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
 ED_Q_MIXED_F32_FLOPS_PER_T, ED_Q_MIXED_F16_FLOPS_PER_T = 7, 8
+ED_Q_MIXED_KERNEL = 9
 
 _lib: Optional[ctypes.CDLL] = None
 

@@ -583,6 +583,16 @@ bool use_h16(const ed_context* c) {
   return use_f16_stage2(c) && ws_fits32(c) && c->mixed_ws == 2 && h16_groups(c) <= 2 && c->V3 >= 4096;
 }
 
+// the contraction kernel the mixed form runs (the order of the tests in
+// run_batches): see ED_Q_MIXED_KERNEL
+int mixed_kernel_id(const ed_context* c) {
+  if (mixed_runs_fp64(c)) return 0;
+  if (!use_f16_stage2(c)) return 1;
+  if (use_h16(c)) return 5;
+  if (use_ws_kernel(c)) return ws_wide(c) ? 4 : 3;
+  return 2;
+}
+
 // Scale exponent of the FP16 stage-1 workspace: every stored component,
 // times the largest split coefficient, stays below 2^15.  Unitary links keep
 // a colour vector's norm, so a hop at most doubles it and a straight path
@@ -929,6 +939,9 @@ struct ed_batch {
   DevBuf<ed::GroupProblem> probs[3];
   std::vector<ed::GroupProblem> host_probs[3];
   int nprob[3] = {0, 0, 0}, total[3] = {0, 0, 0}, nks[3] = {1, 1, 1};
+  // kind 2 runs the 64-bit-address form (some lattice's timeslice workspace
+  // reaches 2^31 bytes, or ED_OPT_WIDE_WS is set on one)
+  bool wide = false;
   std::vector<uint64_t> fingerprint;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -967,12 +980,9 @@ int bguard(ed_batch* b, const std::function<void()>& body) {
   }
 }
 
-// the grouped kernels address each workspace with 32-bit buffer offsets: a
-// lattice whose timeslice workspace reaches 2^31 bytes runs the FP32 phase
-// contraction in a batch
 int batch_kind(const ed_context* c, int mode) {
   if (mode == ED_MODE_FP64 || mixed_runs_fp64(c)) return 0;
-  return use_f16_stage2(c) && ws_fits32(c) ? 2 : 1;
+  return use_f16_stage2(c) ? 2 : 1;
 }
 
 // Plans, workspaces (every timeslice at once) and the grouped problem tables;
@@ -980,6 +990,7 @@ int batch_kind(const ed_context* c, int mode) {
 std::vector<uint64_t> batch_prepare(ed_batch* b) {
   std::vector<ed::GroupProblem> tab[3];
   std::vector<uint64_t> fp;
+  b->wide = false;
   for (auto& it : b->items) {
     ed_context* c = it.ctx;
     require(c->lattice_set && c->gauge_set && c->eig_set && c->mom_set, ED_ERR_STATE,
@@ -1007,6 +1018,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
     gp.work0 = b->total[kind];
     tab[kind].push_back(gp);
     b->nks[kind] = std::max(b->nks[kind], c->nks_max);
+    if (kind == 2 && ws_wide(c)) b->wide = true;
     fp.push_back((uint64_t)(uintptr_t)c);
     fp.push_back(c->gen);
     fp.push_back((uint64_t)(uintptr_t)W);
@@ -1017,6 +1029,10 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
     if (tot >= ((int64_t)1 << 31)) fail(ED_ERR_UNSUPPORTED, "batch has too many work items");
     b->total[kind] = (int)tot;
   }
+  // the grouped kernels' 64-bit-address form covers up to four split terms
+  require(!(b->wide && b->nks[2] > 3), ED_ERR_UNSUPPORTED,
+          "a lattice with 2^31 or more bytes of workspace per timeslice shares the mixed batch with "
+          "derivatives of more than four split terms; batch them separately");
   for (int k = 0; k < 3; ++k) {
     b->nprob[k] = (int)tab[k].size();
     if (tab[k].empty()) continue;
@@ -1034,6 +1050,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
   }
   fp.push_back((uint64_t)b->grid_per_cu);
   fp.push_back((uint64_t)b->order);
+  fp.push_back((uint64_t)b->wide);
   return fp;
 }
 
@@ -1064,17 +1081,21 @@ void batch_enqueue(ed_batch* b) {
     const int np = b->nprob[k], tot = b->total[k], ord = b->order, nks = b->nks[k];
     if (nks > 6) fail(ED_ERR_UNSUPPORTED, "derivative with more split terms than the batched kernels take");
     const int ks = nks <= 1 ? 1 : nks <= 2 ? 2 : nks <= 3 ? 3 : 6;
-#define ED_GROUP(KS)                                                                                    \
-  do {                                                                                                  \
-    if (k == 0) ed::contract_group_kernel<double, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);   \
-    else if (k == 1) ed::contract_group_kernel<float, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord); \
-    else if (b->grid_per_cu <= 0) ed::contract_mixed_item_kernel<KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord); \
-    else ed::contract_mixed_group_kernel<KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);            \
+    const bool wide = k == 2 && b->wide;  // ks <= 3 then (batch_prepare)
+#define ED_GROUP(KS, WD)                                                                                   \
+  do {                                                                                                     \
+    if (k == 0) ed::contract_group_kernel<double, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);      \
+    else if (k == 1) ed::contract_group_kernel<float, KS><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);  \
+    else if (b->grid_per_cu <= 0) ed::contract_mixed_item_kernel<KS, WD><<<grid, blk, 0, b->stream>>>(P, np, tot, ord); \
+    else ed::contract_mixed_group_kernel<KS, WD><<<grid, blk, 0, b->stream>>>(P, np, tot, ord);           \
   } while (0)
-    if (ks == 1) ED_GROUP(1);
-    else if (ks == 2) ED_GROUP(2);
-    else if (ks == 3) ED_GROUP(3);
-    else ED_GROUP(6);
+    if (ks == 1 && wide) ED_GROUP(1, true);
+    else if (ks == 1) ED_GROUP(1, false);
+    else if (ks == 2 && wide) ED_GROUP(2, true);
+    else if (ks == 2) ED_GROUP(2, false);
+    else if (ks == 3 && wide) ED_GROUP(3, true);
+    else if (ks == 3) ED_GROUP(3, false);
+    else ED_GROUP(6, false);
 #undef ED_GROUP
     HIPCHK(hipGetLastError());
   }
@@ -1475,6 +1496,7 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
       case ED_Q_MOMENTUM_BLOCKS: *value = ctx->npb; break;
       case ED_Q_HOPS_PER_T: *value = ctx->disp ? 6 * ctx->distance : ctx->D - 1; break;
       case ED_Q_SPLITS: *value = ctx->total_splits; break;
+      case ED_Q_MIXED_KERNEL: *value = mixed_kernel_id(ctx); break;
       default: fail(ED_ERR_ARG, "unknown query %d", what);
     }
   });

@@ -1542,7 +1542,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_group_kernel(const Gro
   }
 }
 
-template <int NKS>
+template <int NKS, bool WIDE = false>
 __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(const GroupProblem* __restrict__ probs,
                                                                              int nprob, int total, int order) {
   __shared__ float2 Qs[2][SC2][QSITE];
@@ -1557,8 +1557,8 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(con
     const ContractArgs a = probs[i].a;
     __syncthreads();
     if constexpr (NKS <= 3)
-      contract_mixed_ws_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
-                                  item - probs[i].work0, Qs, wmax, PHs);
+      contract_mixed_ws_body<NKS, WIDE>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                                        item - probs[i].work0, Qs, wmax, PHs);
     else
       contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
                                item - probs[i].work0, (float2(*)[SC2][16][16])(void*)Qs, wmax);
@@ -1568,7 +1568,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_group_kernel(con
 // One workgroup per work item of the problem table (the batch's launch shape
 // "one workgroup per item"): no persistent loop, so no state lives across
 // items.  Grid = 8 ceil(total / 8); item mapping as contract_mixed_group_kernel.
-template <int NKS>
+template <int NKS, bool WIDE = false>
 __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_item_kernel(const GroupProblem* __restrict__ probs,
                                                                             int nprob, int total, int order) {
   __shared__ float2 Qs[2][SC2][QSITE];
@@ -1579,8 +1579,8 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_item_kernel(cons
   const int i = __builtin_amdgcn_readfirstlane(group_find(probs, nprob, item));  // uniform: scalar loads
   const ContractArgs a = probs[i].a;
   if constexpr (NKS <= 3)
-    contract_mixed_ws_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
-                                item - probs[i].work0, Qs, wmax, PHs);
+    contract_mixed_ws_body<NKS, WIDE>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                                      item - probs[i].work0, Qs, wmax, PHs);
   else
     contract_mixed_body<NKS>((const float2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
                              item - probs[i].work0, (float2(*)[SC2][16][16])(void*)Qs, wmax);

@@ -180,6 +180,11 @@ int ed_profile_reset(ed_context* ctx);
  *   ED_Q_SPLITS                split terms over all derivatives (S)
  *   ED_Q_MIXED_F32_FLOPS_PER_T FP32 MFMA flops of the mixed FP16-stage-2 kernel per timeslice
  *   ED_Q_MIXED_F16_FLOPS_PER_T FP16 MFMA flops of the same kernel per timeslice
+ *   ED_Q_MIXED_KERNEL          the contraction the mixed form runs: 0 the FP64 kernels
+ *                              (V3 < 4096), 1 FP32 phase contraction, 2 FP16 phase contraction
+ *                              with uniform waves, 3 warp-specialized (32-bit workspace
+ *                              offsets), 4 warp-specialized with 64-bit addresses, 5 FP16
+ *                              stage 1
  * (ED_Q_CONTRACT_FLOPS_PER_T counts the FP64 / FP32-stage-2 kernels.) */
 enum ed_query {
   ED_Q_CONTRACT_FLOPS_PER_T = 1,
@@ -189,7 +194,8 @@ enum ed_query {
   ED_Q_HOPS_PER_T = 5,
   ED_Q_SPLITS = 6,
   ED_Q_MIXED_F32_FLOPS_PER_T = 7,
-  ED_Q_MIXED_F16_FLOPS_PER_T = 8
+  ED_Q_MIXED_F16_FLOPS_PER_T = 8,
+  ED_Q_MIXED_KERNEL = 9
 };
 int ed_query(ed_context* ctx, int what, int64_t* value);
 

@@ -320,3 +320,25 @@ def test_mixed_below_4096_sites_runs_fp64_kernels():
     # the forced FP32 kernel on a lattice this small sits near the mixed bound
     # (the reason for the routing); here it only has to be the mixed kernel
     assert blockwise(Em32, E64) <= 1e-5
+
+
+def test_mixed_kernel_query_follows_the_routing():
+    """ED_Q_MIXED_KERNEL names the contraction the mixed form runs for the settings."""
+    U, V = S.make_inputs_np([6, 5, 7, 1], 4, "haar", seed=2)
+    cases = [
+        ([6, 5, 7, 1], {}, 0),
+        ([6, 5, 7, 1], {_native.ED_OPT_MIXED_STAGE2: 0}, 1),
+        ([6, 5, 7, 1], {_native.ED_OPT_MIXED_STAGE2: 1, _native.ED_OPT_MIXED_WS: 0}, 2),
+        ([16, 16, 16, 1], {}, 3),
+        ([16, 16, 16, 1], {_native.ED_OPT_WIDE_WS: 1}, 4),
+        ([16, 16, 16, 1], {_native.ED_OPT_MIXED_WS: 2}, 5),
+    ]
+    for latt, opts, want in cases:
+        g = ed.ElementalGenerator(latt, None, None, 2, S.momenta_by_norm(2), Ne=4, mode="mixed")
+        for k, v in opts.items():
+            g.set_option(k, v)
+        Ui, Vi = (U, V) if latt[0] == 6 else S.make_inputs_np(latt, 4, "haar", seed=3)
+        g.load_arrays(Ui, Vi)
+        got = g.context.query(_native.ED_Q_MIXED_KERNEL)
+        say(f"mixed kernel for {latt} {opts}: {got}")
+        assert got == want

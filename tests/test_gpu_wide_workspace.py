@@ -58,6 +58,7 @@ def test_wide_addresses_match_32bit_offsets(case):
         g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
         g.set_option(_native.ED_OPT_WIDE_WS, wide)
         g.load_arrays(U, V)
+        assert g.context.query(_native.ED_Q_MIXED_KERNEL) == (4 if wide else 3)
         outs.append(g.calc_all())
     err = blockwise(outs[1], O.elemental(V, U, latt, nn, moms))
     say(f"  identical: {np.array_equal(outs[0], outs[1])}, vs oracle {err:.2e}")
@@ -81,6 +82,8 @@ def test_mixed_past_2gib_per_timeslice():
     for mode in ("mixed", "fp64"):
         g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode=mode)
         g.load_arrays(U, V)
+        if mode == "mixed":
+            assert g.context.query(_native.ED_Q_MIXED_KERNEL) == 4
         outs[mode] = g.calc_all(out=torch.empty((1, D, P, Ne, Ne), dtype=torch.complex128, device=dev))
         torch.cuda.synchronize()
         del g
@@ -93,3 +96,58 @@ def test_mixed_past_2gib_per_timeslice():
     say(f"  mixed vs fp64 blockwise {err:.2e}")
     assert err <= MIXED_TOL
     assert float(r.max()) > 0
+
+
+def test_batch_wide_form_equals_single_lattices():
+    """A batch holding a lattice past 2^31 bytes per timeslice (and one forced
+    wide) runs the grouped kernels' 64-bit-address form: every output equals
+    the lattice's single-lattice output bit for bit, for both launch shapes."""
+    import torch
+
+    from easydistillation_amd.batch import ElementalBatch
+
+    dev = torch.device("cuda:0")
+    moms = S.momenta_by_norm(4)
+    gens = []
+    big = [32, 32, 32, 1]
+    U = S.gauge_torch(big, "smooth", 0.3, seed=7, device=dev)
+    V = S.eigvecs_torch(big, 256, seed=8, device=dev)
+    g = ed.ElementalGenerator(big, None, None, 2, moms, Ne=256, mode="mixed")
+    g.load_arrays(U, V)
+    gens.append(g)
+    del U, V
+    for i, (latt, Ne, wide) in enumerate((([16, 16, 16, 2], 21, 1), ([16, 16, 18, 1], 40, 0))):
+        U, V = S.make_inputs_np(latt, Ne, "haar", seed=40 + i)
+        g = ed.ElementalGenerator(latt, None, None, 2, moms, Ne=Ne, mode="mixed")
+        g.set_option(_native.ED_OPT_WIDE_WS, wide)
+        g.load_arrays(U, V)
+        gens.append(g)
+    singles = [g.calc_all(out=torch.empty((g.latt_size[3], 13, len(moms), g.Ne, g.Ne), dtype=torch.complex128,
+                                          device=dev)) for g in gens]
+    b = ElementalBatch(gens)
+    for grid in (0, 2):
+        b.set_option(_native.ED_BOPT_GRID_PER_CU, grid)
+        outs = b.run()
+        torch.cuda.synchronize()
+        same = [bool(torch.equal(o, s)) for o, s in zip(outs, singles)]
+        say(f"batch with a 2.6 GB-per-timeslice lattice, grid/CU={grid}: identical {same}")
+        assert all(same)
+
+
+def test_batch_wide_form_with_long_split_lists_is_refused():
+    """The grouped kernels take derivatives of more than four split terms
+    (three derivatives) only in the 32-bit form: a batch that needs both
+    raises instead of running."""
+    from easydistillation_amd.batch import ElementalBatch
+
+    moms = [(0, 0, 0), (1, 0, 0)]
+    gens = []
+    for latt, Ne, nn, wide in (([16, 16, 16, 1], 8, 1, 1), ([16, 16, 16, 1], 8, 3, 0)):
+        U, V = S.make_inputs_np(latt, Ne, "haar", seed=11)
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
+        g.set_option(_native.ED_OPT_WIDE_WS, wide)
+        g.load_arrays(U, V)
+        gens.append(g)
+    b = ElementalBatch(gens)
+    with pytest.raises(_native.NativeError, match="split terms"):
+        b.run()
