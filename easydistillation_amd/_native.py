@@ -15,6 +15,7 @@ LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libelemental_hip.so")
 
 ED_OK = 0
+ED_ERR_ARG, ED_ERR_STATE, ED_ERR_HIP, ED_ERR_ALLOC, ED_ERR_UNSUPPORTED = -1, -2, -3, -4, -5
 ED_C128, ED_C64 = 0, 1
 ED_MODE_FP64, ED_MODE_MIXED = 0, 1
 ED_OPT_HERMITIAN, ED_OPT_TBATCH, ED_OPT_WS_MB, ED_OPT_PROFILE, ED_OPT_MIXED_STAGE2 = 1, 2, 3, 4, 5

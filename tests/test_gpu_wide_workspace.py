@@ -149,5 +149,6 @@ def test_batch_wide_form_with_long_split_lists_is_refused():
         g.load_arrays(U, V)
         gens.append(g)
     b = ElementalBatch(gens)
-    with pytest.raises(_native.NativeError, match="split terms"):
+    with pytest.raises(_native.NativeError, match="split terms") as err:
         b.run()
+    assert err.value.code == _native.ED_ERR_UNSUPPORTED

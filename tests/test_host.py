@@ -29,6 +29,18 @@ def test_library_exports_every_header_symbol():
     assert lib.ed_version().startswith(b"elemental_hip")
 
 
+def test_binding_constants_match_header_enums():
+    """Every ED_* enum value the header declares has the same value in the Python binding."""
+    text = open(HEADER).read()
+    values = {}
+    for body in re.findall(r"enum\s+\w+\s*\{(.*?)\}", text, re.S):
+        for name, val in re.findall(r"(ED_\w+)\s*=\s*(-?\d+)", body):
+            values[name] = int(val)
+    assert {"ED_OPT_WIDE_WS", "ED_Q_MIXED_KERNEL", "ED_KERNEL_CONTRACT"} <= set(values)
+    for name, val in values.items():
+        assert getattr(_native, name) == val, name
+
+
 def test_no_silent_cpu_fallback():
     try:
         import torch
