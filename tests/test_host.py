@@ -215,3 +215,13 @@ def test_momentum_dict_orders_match_the_reference_tables():
     assert len(mom_dict_to_list(9)) == len(ball) == 123 and set(mom_dict_to_list(9)) == ball
     with pytest.raises(ValueError):
         momentum_dict(5)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
