@@ -553,8 +553,8 @@ size_t per_t_ws_bytes(const ed_context* c) {
 bool ws_fits32(const ed_context* c) { return per_t_ws_bytes<float>(c) < ((size_t)1 << 31); }
 
 // the warp-specialized FP32 stage-1 kernel, which also has a 64-bit-address
-// form for larger workspaces
-bool use_ws_kernel(const ed_context* c) { return c->mixed_ws != 0 && c->nks_max <= 3; }
+// form for larger workspaces (where it replaces the uniform kind too)
+bool use_ws_kernel(const ed_context* c) { return (c->mixed_ws != 0 || !ws_fits32(c)) && c->nks_max <= 3; }
 bool ws_wide(const ed_context* c) { return !ws_fits32(c) || c->wide_ws; }
 
 // the mixed form takes the FP16-MFMA phase contraction when chosen and its

@@ -142,8 +142,9 @@ int ed_synchronize(ed_context* ctx);
  *                         colour) terms than the chosen kernel takes use the latter.
  *                         Kinds 0 and 2 address one timeslice's workspace with 32-bit
  *                         offsets; from 2^31 bytes per timeslice (e.g. 32^3 with Nev 256 and
- *                         two derivatives) kind 2 runs as kind 1, which has a 64-bit-address
- *                         form, and kind 0 as the FP32 phase contraction.
+ *                         two derivatives) kinds 0 and 2 run as kind 1, which has a
+ *                         64-bit-address form (derivatives of more than four split terms then
+ *                         take the FP32 phase contraction).
  *   ED_OPT_WIDE_WS        1: kind 1 uses its 64-bit-address form below 2^31 bytes too (same
  *                         results bit for bit; for testing). */
 enum ed_option {

@@ -74,6 +74,12 @@ def test_mixed_past_2gib_per_timeslice():
     D, P = ed.num_derivative(nn), len(moms)
     # slots x padded Nev x colours x sites x complex64
     assert 13 * Ne * 3 * 32**3 * 8 >= 1 << 31
+    # the 32-bit-only kinds give way to the 64-bit warp-specialized kernel here
+    g0 = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
+    for kind in (0, 1, 2):
+        g0.set_option(_native.ED_OPT_MIXED_WS, kind)
+        assert g0.context.query(_native.ED_Q_MIXED_KERNEL) == 4
+    del g0
     dev = torch.device("cuda:0")
     say(f"past 2 GiB: latt={latt} Ne={Ne} nabla={nn} P={P}")
     U = S.gauge_torch(latt, "smooth", 0.3, seed=5, device=dev)
