@@ -180,7 +180,7 @@ struct ed_context {
 
   // workspace
   int tbatch_opt = 0;
-  int64_t ws_mb = 36864;
+  int64_t ws_mb = 65536;  // 8 timeslices of 7.2 GB (48^3, Nev 200) fit; 288 GB of HBM
   DevBuf<char> W;
   DevBuf<double2> stage;
 
