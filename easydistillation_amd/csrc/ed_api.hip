@@ -625,7 +625,12 @@ template <typename T>
 int auto_tbatch(const ed_context* c, int nt) {
   int TB = c->tbatch_opt;
   if (TB <= 0) {
-    const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    // at most this context's current workspace (reused) plus three quarters
+    // of the free device memory, so a shared or crowded GPU gets smaller
+    // batches instead of a failed allocation
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = std::min(budget, c->W.n + free_b / 4 * 3);
     const size_t per_t = per_t_ws_bytes<T>(c);
     TB = (int)std::max<size_t>(1, budget / per_t);
     // whole timeslices per XCD: the contraction hands each of the 8 XCDs a

@@ -121,7 +121,8 @@ int ed_synchronize(ed_context* ctx);
  *                         E_{rev d}[p] = (-1)^{|d|} E_d[-p]^dagger; 0: compute all d.
  *   ED_OPT_TBATCH         timeslices per batch (0 = automatic: a multiple of 8 when 8 fit the
  *                         budget, exactly 8 when a timeslice needs >= 1 GiB of workspace).
- *   ED_OPT_WS_MB          workspace budget in MiB for the automatic batch (default 65536).
+ *   ED_OPT_WS_MB          workspace budget in MiB for the automatic batch (default 65536; never
+ *                         more than the context's workspace plus 3/4 of the free device memory).
  *   ED_OPT_PROFILE        1: record HIP events around every kernel launch.
  *   ED_OPT_MIXED_STAGE2   mixed form's phase contraction: 1 FP16 MFMA on a three-term
  *                         hi/lo split, 0 FP32 MFMA, -1 (default) FP16 when V3 >= 4096; with
