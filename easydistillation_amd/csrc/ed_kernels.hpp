@@ -1271,7 +1271,6 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
                                                        float2 (*Qs)[SC2][QSITE], float (*wmax)[NWAVE],
                                                        half8 (*PHs)[4][64]) {
   typedef float4v acc_t;
-  typedef float v2f __attribute__((ext_vector_type(2)));
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int l16 = lane & 15, g4 = lane >> 4;

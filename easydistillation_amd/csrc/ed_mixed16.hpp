@@ -302,7 +302,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     }
   }
   for (int chunk = 0; chunk < nchunk; ++chunk) {
-    const int c0 = chunk * SC2, buf = chunk & 1;
+    const int buf = chunk & 1;
     float tmax = 0.f;
 #pragma unroll
     for (int v = 0; v < NPROD; ++v) tmax = fmaxf(tmax, wmax[buf][v]);
