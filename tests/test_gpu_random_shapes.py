@@ -4,8 +4,9 @@ batches of random lattices against the single-lattice outputs.
 Lattice extents 2..9 (and 16^3 with odd extents in the last axis), Ne 1..40,
 num_nabla 0..2, random momentum lists with repeats, rough and smooth links.
 Bars: FP64 <= 1e-12; mixed <= 1e-6 (below V3 = 4096 the mixed form runs the
-FP64 kernels); batch outputs equal the single-lattice outputs bit for bit for
-every launch shape.
+FP64 kernels; on the 16^3 cases its 64-bit-address form equals it bit for
+bit); batch outputs equal the single-lattice outputs bit for bit for every
+launch shape.
 """
 
 import sys
@@ -50,18 +51,27 @@ def random_spec(rng, allm, big):
 @pytest.mark.parametrize("case", range(12))
 def test_random_shape_against_oracle(case):
     rng = np.random.default_rng(1000 + case)
-    latt, Ne, nn, moms, links = random_spec(rng, S.momenta_by_norm(5), big=case % 4 == 3)
+    big = case % 4 == 3
+    latt, Ne, nn, moms, links = random_spec(rng, S.momenta_by_norm(5), big=big)
     U, V = S.make_inputs_np(latt, Ne, links, seed=int(rng.integers(0, 10**6)))
     ref = O.elemental(V, U, latt, nn, moms)
-    errs = {}
+    errs, outs = {}, {}
     for mode in ("fp64", "mixed"):
         g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode=mode)
         g.load_arrays(U, V)
-        errs[mode] = blockwise(g.calc_all(), ref)
+        outs[mode] = g.calc_all()
+        errs[mode] = blockwise(outs[mode], ref)
     say(f"random {case}: latt={latt} Ne={Ne} nabla={nn} P={len(moms)} {links}: "
         f"fp64 {errs['fp64']:.1e} mixed {errs['mixed']:.1e}")
     assert errs["fp64"] <= 1e-12
     assert errs["mixed"] <= 1e-6
+    if big:
+        # the 64-bit-address form of the mixed kernel gives the same outputs
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
+        g.set_option(_native.ED_OPT_WIDE_WS, 1)
+        g.load_arrays(U, V)
+        assert g.context.query(_native.ED_Q_MIXED_KERNEL) == 4
+        assert np.array_equal(g.calc_all(), outs["mixed"])
 
 
 @pytest.mark.parametrize("case", range(4))
