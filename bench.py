@@ -64,6 +64,8 @@ BATCHED = {
     "D": [([16, 16, 16, 32], 64), ([24, 24, 24, 48], 128), ([32, 32, 32, 64], 256)],
     "D-small": [([8, 8, 8, 4], 16), ([12, 12, 12, 6], 32), ([16, 16, 16, 8], 64)],
 }
+# differentiation form: lattice, Nev, num_nabla (4 derivative combinations), the 7 momenta |n|^2 <= 1
+DIFFERENTIATION = {"E": ([24, 24, 24, 64], 128, 1)}
 
 
 def log(msg):
@@ -150,6 +152,107 @@ def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
             f"numpy {np.__version__}, {threads} threads, {platform.processor() or platform.machine()}"
         ),
     }
+
+
+def bench_differentiation(args):
+    """Differentiation form: L = sum_{t,d,p} |sum_k w_k E[t,d,p,k,k]|^2 and dL/dU, dL/dV, dL/dw.
+
+    One step = the forward FP64 form over every timeslice plus the backward
+    through the hand-written adjoint (ed_backward, which recomputes the
+    displaced vectors) and the profile-trace adjoint.  value = Lt P D Ne^2 /
+    seconds per step.  The roofline object covers the contraction adjoint
+    (adj_contract_kernel, FP64 vector arithmetic): per (t, site, row, column)
+    a phase sum of 2 P complex products (16 P flops) for each of the D
+    derivatives and, per split term and colour, two complex products and
+    their scaled accumulation (20 flops): V3 Ne^2 (16 P D + 60 S) flops per
+    timeslice, S = sum over derivatives of 2^|d| split terms.  The reference
+    has no reverse pass, so there is no CPU baseline.
+    """
+    import torch
+
+    from easydistillation_amd import _native
+    from easydistillation_amd import synthetic as S
+    from easydistillation_amd.autograd import ElementalEngine, profiled_loss
+    from easydistillation_amd.insertion import derivative, num_derivative
+
+    dev = torch.device("cuda:0")
+    latt, Ne, num_nabla = DIFFERENTIATION[args.workload]
+    Lx, Ly, Lz, Lt = latt
+    moms = S.unit_momenta()
+    P, D = len(moms), num_derivative(num_nabla)
+    nsplit = sum(2 ** len(derivative(d)) for d in range(D))
+    t0 = time.time()
+    U = S.gauge_torch(latt, "haar", seed=0, device=dev)
+    V = S.eigvecs_torch(latt, Ne, seed=1, device=dev, log=log).to(torch.complex128)
+    gen = torch.Generator(device=dev).manual_seed(2)
+    w = torch.rand(Ne, dtype=torch.float64, device=dev, generator=gen) + 0.5
+    log(f"inputs in {time.time() - t0:.1f} s")
+    eng = ElementalEngine(latt, Ne, num_nabla, moms)
+    Ug, Vg, wg = (x.clone().requires_grad_() for x in (U, V, w))
+    del U, V
+
+    def step():
+        loss = profiled_loss(Ug, Vg, wg, eng)
+        return torch.autograd.grad(loss, [Ug, Vg, wg])
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    eng.ctx.set_option(_native.ED_OPT_PROFILE, 1)
+    eng.ctx.profile_reset()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        log(f"differentiation step {i} done at {time.perf_counter() - t0:.2f} s")
+    dt = (time.perf_counter() - t0) / args.steps
+    eng.ctx.set_option(_native.ED_OPT_PROFILE, 0)
+    a_ms, a_n = eng.ctx.profile_read(_native.ED_KERNEL_ADJOINT)
+    c_ms, c_n = eng.ctx.profile_read(_native.ED_KERNEL_CONTRACT)
+    h_ms, _ = eng.ctx.profile_read(_native.ED_KERNEL_STENCIL)
+    adj_flops = float(Lx * Ly * Lz) * Ne * Ne * (16 * P * D + 60 * nsplit) * Lt * args.steps / max(a_n, 1)
+    a_avg = a_ms / max(a_n, 1)
+    achieved = adj_flops / (a_avg * 1e-3) / 1e12
+    line = {
+        "metric": "elementals/s through forward + backward (dL/dU, dL/dV, dL/dw), FP64, 1 GPU",
+        "value": Lt * P * D * Ne * Ne / dt,
+        "unit": "elementals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Haar SU(3) links, orthonormal eigenvectors, weights U(0.5, 1.5))",
+        "config": {
+            "workload": f"{Lx}^3x{Lt}" if Lx == Ly == Lz else f"{Lx}x{Ly}x{Lz}x{Lt}",
+            "Nev": Ne,
+            "momenta": P,
+            "derivatives": D,
+            "form": "differentiation",
+            "elementals_per_step": Lt * P * D * Ne * Ne,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK["fp64"],
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK["fp64"],
+            "traffic": None,
+            "kernel": "adj_contract_kernel",
+            "avg_launch_ms": a_avg,
+            "flops_per_launch": adj_flops,
+            "flops_basis": "V3 Ne^2 (16 P D + 60 S) FP64 flops per timeslice (phase sums and split-operand gradients)",
+        },
+        "forward_contract_ms_per_step": c_ms / args.steps,
+        "stencil_ms_per_step": h_ms / args.steps,
+        "adjoint_contract_ms_per_step": a_ms / args.steps,
+        "cpu_baseline": None,
+        "cpu_baseline_note": "the reference has no reverse pass",
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_batched(args):
@@ -280,7 +383,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS) + sorted(BATCHED))
+    ap.add_argument("--workload", default="C", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(DIFFERENTIATION))
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--mixed-ws", type=int, default=1, choices=(0, 1, 2),
                     help="mixed form's FP16 stage-2 kernel: 1 warp-specialized with FP32 stage 1 (default), "
@@ -306,6 +409,9 @@ def main():
         )
         sys.exit(2)
 
+    if args.workload in DIFFERENTIATION:
+        bench_differentiation(args)
+        return
     if args.workload in BATCHED:
         bench_batched(args)
         return

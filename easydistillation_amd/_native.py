@@ -22,7 +22,7 @@ ED_OPT_HERMITIAN, ED_OPT_TBATCH, ED_OPT_WS_MB, ED_OPT_PROFILE, ED_OPT_MIXED_STAG
 ED_OPT_EXACT_V = 6
 ED_OPT_MIXED_WS = 7
 ED_OPT_WIDE_WS = 8
-ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE = 0, 1, 2
+ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE, ED_KERNEL_ADJOINT = 0, 1, 2, 3
 
 # Every exported symbol and its ctypes signature (restype, argtypes).
 _VP = ctypes.c_void_p

@@ -186,8 +186,8 @@ struct ed_context {
 
   // profiling
   int profile = 0;
-  double prof_ms[3] = {0, 0, 0};
-  int64_t prof_n[3] = {0, 0, 0};
+  double prof_ms[4] = {0, 0, 0, 0};
+  int64_t prof_n[4] = {0, 0, 0, 0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
 
@@ -855,7 +855,7 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
     a.splits = c->bsplits.p;
     a.dsplit = c->bdsplit.p;
     a.phase = c->phase_all.p;
-    {
+    c->launch(ED_KERNEL_ADJOINT, [&] {
       dim3 grid((c->V3 + 7) / 8, (Ne + 63) / 64, nb);
       if (c->bns_max <= 1)
         ed::adj::adj_contract_kernel<1><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
@@ -867,8 +867,7 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
         ed::adj::adj_contract_kernel<8><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
       else
         fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", c->bns_max);
-      HIPCHK(hipGetLastError());
-    }
+    });
     // hop adjoint, deepest level first
     for (int k = c->num_nabla; k >= 1; --k) {
       const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
@@ -897,6 +896,10 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
                               hipMemcpyDeviceToHost, c->stream));
     }
     if (!o_dev || !g_dev) HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (c->profile) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->drain_events();
   }
 }
 
@@ -1519,7 +1522,7 @@ int ed_synchronize(ed_context* ctx) {
 int ed_profile_read(ed_context* ctx, int kernel, double* total_ms, int64_t* launches) {
   if (!ctx) return ED_ERR_ARG;
   return guard(ctx, [&] {
-    if (kernel < 0 || kernel > 2) fail(ED_ERR_ARG, "unknown kernel family %d", kernel);
+    if (kernel < 0 || kernel > 3) fail(ED_ERR_ARG, "unknown kernel family %d", kernel);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->drain_events();
     if (total_ms) *total_ms = ctx->prof_ms[kernel];
@@ -1532,7 +1535,7 @@ int ed_profile_reset(ed_context* ctx) {
   return guard(ctx, [&] {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->drain_events();
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       ctx->prof_ms[k] = 0;
       ctx->prof_n[k] = 0;
     }

@@ -163,10 +163,11 @@ int ed_set_option(ed_context* ctx, int option, int64_t value);
 /* Kernel timing collected while ED_OPT_PROFILE = 1, per kernel family:
  *   ED_KERNEL_STENCIL  covariant hops (one launch per derivative level),
  *   ED_KERNEL_CONTRACT the two-stage contraction,
- *   ED_KERNEL_STAGE    staging of V into the workspace.
+ *   ED_KERNEL_STAGE    staging of V into the workspace,
+ *   ED_KERNEL_ADJOINT  the contraction adjoint of ed_backward.
  * total_ms is the sum of HIP-event durations on the launch stream and
  * launches their number; both are reset by ed_profile_reset. */
-enum ed_kernel { ED_KERNEL_STENCIL = 0, ED_KERNEL_CONTRACT = 1, ED_KERNEL_STAGE = 2 };
+enum ed_kernel { ED_KERNEL_STENCIL = 0, ED_KERNEL_CONTRACT = 1, ED_KERNEL_STAGE = 2, ED_KERNEL_ADJOINT = 3 };
 int ed_profile_read(ed_context* ctx, int kernel, double* total_ms, int64_t* launches);
 int ed_profile_reset(ed_context* ctx);
 
