@@ -17,9 +17,11 @@ from .preset import (
     EigenvectorTimeSlice,
     ElementalBinary,
     ElementalNpy,
+    GaugeFieldBinary,
     GaugeFieldHostmem,
     GaugeFieldIldg,
     GaugeFieldNpy,
+    GaugeFieldTimeSlice,
     save_elemental,
 )
 
@@ -43,8 +45,10 @@ __all__ = [
     "EigenvectorTimeSlice",
     "ElementalBinary",
     "ElementalNpy",
+    "GaugeFieldBinary",
     "GaugeFieldIldg",
     "GaugeFieldHostmem",
     "GaugeFieldNpy",
+    "GaugeFieldTimeSlice",
     "save_elemental",
 ]

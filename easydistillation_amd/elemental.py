@@ -138,12 +138,21 @@ class ElementalGenerator:
     # ------------------------------------------------------------------ input
     def load(self, key: str):
         """Read links and eigenvectors of configuration ``key`` (elemental.py:102-105)."""
-        gauge = self.gauge_field.load(key)
-        U = np.asarray(gauge[:])
         Lx, Ly, Lz, Lt = self.latt_size
-        if U.size != Lt * Lz * Ly * Lx * Nd * Nc * Nc:
-            raise ValueError(f"gauge field shape {U.shape} is not [Lt,Lz,Ly,Lx,Nd,Nc,Nc]")
-        U = U.reshape(Lt, Lz, Ly, Lx, Nd, Nc, Nc).transpose(4, 0, 1, 2, 3, 5, 6)[: Nd - 1]
+        gauge = self.gauge_field.load(key)
+        by_record = getattr(self.gauge_field, "spatial_links", None)
+        if by_record is not None:
+            # containers laid out per (t, mu) assemble the spatial links themselves
+            U = by_record(key, self.latt_size)
+        else:
+            U = np.asarray(gauge[:])
+            n = Lt * Lz * Ly * Lx * Nd * Nc * Nc
+            if U.dtype.kind == "f" and U.size == 2 * n:
+                # a real array of interleaved (re, im) pairs
+                U = np.ascontiguousarray(U, dtype=np.float64).view(np.complex128)
+            if U.size != n:
+                raise ValueError(f"gauge field shape {U.shape} is not [Lt,Lz,Ly,Lx,Nd,Nc,Nc]")
+            U = U.reshape(Lt, Lz, Ly, Lx, Nd, Nc, Nc).transpose(4, 0, 1, 2, 3, 5, 6)[: Nd - 1]
         self._gauge_field_path = getattr(gauge, "file", None)
         eig = self.eigenvector.load(key)
         V = np.empty((Lt, self.Ne, Lz, Ly, Lx, Nc), dtype=np.complex64)

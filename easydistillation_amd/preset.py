@@ -8,9 +8,14 @@ lattice/filedata/abstract.py:12-20):
 
 The classes here provide that contract for numpy ``.npy`` files (read through
 a memory map, so only the slices asked for are read), for arrays already in
-memory, for ILDG gauge configurations (``GaugeFieldIldg``,
-lattice/preset.py:151-160) and for QDP LazyDiskMapObj eigenvector files
-(``EigenvectorTimeSlice``, lattice/preset.py:55-64).  ``ElementalNpy`` and
+memory, for gauge configurations as ILDG files (``GaugeFieldIldg``,
+lattice/preset.py:151-160), QDP LazyDiskMapObj files with one record per
+(t, mu) (``GaugeFieldTimeSlice``, lattice/preset.py:44-53) or headerless
+binary arrays (``GaugeFieldBinary``, lattice/preset.py:162-170), and for QDP
+LazyDiskMapObj eigenvector files (``EigenvectorTimeSlice``,
+lattice/preset.py:55-64).  A gauge container whose layout is not
+[Lt, Lz, Ly, Lx, Nd, Nc, Nc] provides ``spatial_links(key, latt_size)``,
+which ``ElementalGenerator.load`` uses instead of ``load(key)[:]``.  ``ElementalNpy`` and
 ``ElementalBinary`` read elementals in the reference's file layout
 [D, P, Lt, Ne, Ne] (lattice/preset.py:173-181, 129-137: .npy, and headerless
 little-endian complex128); ``save_elemental`` writes either.
@@ -22,6 +27,7 @@ from typing import List, Sequence
 
 import numpy as np
 
+from .constant import Nc, Nd
 from .filedata import (BinaryFile, BinaryFileData, IldgFile, IldgFileData, QDPLazyDiskMapObjFile,
                        QDPLazyDiskMapObjFileData, write_binary)
 
@@ -137,6 +143,55 @@ class GaugeFieldIldg(IldgFile, GaugeField):
         self.dtype = dtype
 
     def load(self, key: str) -> IldgFileData:
+        return self.get_file_data(f"{self.prefix}{key}{self.suffix}", self.shape, self.dtype)
+
+
+class GaugeFieldTimeSlice(QDPLazyDiskMapObjFile, GaugeField):
+    """QDP LazyDiskMapObj gauge files with one record per (t, mu) (lattice/preset.py:44-53).
+
+    ``shape`` = [Lt, Nd, *per-record shape], the record being the links of
+    direction mu on timeslice t, [Lz*Ly*Lx, Nc, Nc] (row a, column b, x
+    fastest); records are stored big-endian complex128 and read back as
+    complex64, as the reference's LazyDiskMapObj reader returns them.
+    """
+
+    def __init__(self, prefix: str, suffix: str | None, shape: Sequence[int] = (128, 4, 16**3, 3, 3)):
+        QDPLazyDiskMapObjFile.__init__(self)
+        self.prefix = prefix
+        self.suffix = ".stout.n20.f0.12.mod" if suffix is None else suffix
+        self.shape = list(shape)
+
+    def load(self, key: str) -> QDPLazyDiskMapObjFileData:
+        return self.get_file_data(f"{self.prefix}{key}{self.suffix}", self.shape, ">c16", 2)
+
+    def spatial_links(self, key: str, latt_size: Sequence[int]) -> np.ndarray:
+        """U[3, Lt, Lz, Ly, Lx, 3, 3] from the records (t, mu), mu = x, y, z."""
+        Lx, Ly, Lz, Lt = (int(v) for v in latt_size)
+        data = self.load(key)
+        if data.extra_shape[0] != Lt or int(np.prod(data.shape)) != Lz * Ly * Lx * Nc * Nc:
+            raise ValueError(f"records {data.extra_shape} x {data.shape} do not hold a {latt_size} lattice")
+        U = np.empty((Nd - 1, Lt, Lz, Ly, Lx, Nc, Nc), dtype=np.complex64)
+        for t in range(Lt):
+            for mu in range(Nd - 1):
+                U[mu, t] = data[t, mu].reshape(Lz, Ly, Lx, Nc, Nc)
+        return U
+
+
+class GaugeFieldBinary(BinaryFile, GaugeField):
+    """Gauge configuration as a headerless array ``{prefix}{key}{suffix}``
+    (lattice/preset.py:162-170) of ``shape`` (e.g. [Lt, Lz*Ly*Lx, Nd, Nc, Nc])
+    in ``dtype``.  A real dtype holds interleaved (re, im) pairs, so its last
+    axis is 2 (e.g. shape [Lt, V3, Nd, Nc, Nc, 2], dtype "<f8")."""
+
+    def __init__(self, prefix: str, suffix: str | None, shape: Sequence[int] = (128, 16**3, 4, 3, 3),
+                 dtype: str = "<f8"):
+        BinaryFile.__init__(self)
+        self.prefix = prefix
+        self.suffix = ".dat" if suffix is None else suffix
+        self.shape = list(shape)
+        self.dtype = dtype
+
+    def load(self, key: str) -> BinaryFileData:
         return self.get_file_data(f"{self.prefix}{key}{self.suffix}", self.shape, self.dtype)
 
 

@@ -190,6 +190,7 @@ def elemental_timeslice(
     momentum_list: Sequence[Sequence[int]],
     stocastic_coeff: np.ndarray | None = None,
     round_v_to_c64: bool = True,
+    derivative_indices: Sequence[int] | None = None,
 ) -> np.ndarray:
     """E_t[d, p, e, f] for one timeslice (elemental.py:290-338), complex128.
 
@@ -197,6 +198,9 @@ def elemental_timeslice(
     buffer (elemental.py:55,297-298); everything after is complex128.
     The per-momentum contraction "zyx,ezyxc,fzyxc->ef" is evaluated as a
     complex GEMM (BLAS) over the flattened (site, colour) index.
+    ``derivative_indices`` restricts the output to those derivatives (in the
+    given order; the default is all of them), for spot checks on lattices
+    where the whole timeslice would take minutes.
     """
     V = np.asarray(V_t)
     if round_v_to_c64:
@@ -205,6 +209,8 @@ def elemental_timeslice(
     U_t = np.asarray(U_t, dtype=np.complex128)
     Ne = V.shape[0]
     dl = derivative_list(num_nabla)
+    if derivative_indices is not None:
+        dl = [dl[i] for i in derivative_indices]
     P = len(momentum_list)
     phases = [momentum_phase(latt_size, m) for m in momentum_list]
     E = np.zeros((len(dl), P, Ne, Ne), dtype=np.complex128)

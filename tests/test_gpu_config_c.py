@@ -1,6 +1,15 @@
-"""Headline shape: 32^3, Nev = 200, 33 momenta, 13 derivatives (one timeslice),
-checked through properties that hold at any size (the oracle takes minutes
-here; it pins the same kernels on smaller lattices):
+"""Headline shape: 32^3, Nev = 200, 33 momenta, 13 derivatives.
+
+``test_config_c_full_launch_against_oracle`` runs eight timeslices, which the
+library contracts in one automatic launch (about 17 GB of workspace, each
+timeslice's block just under 2^31 bytes, so the last one sits at the highest
+buffer offsets), in both forms, and checks blocks of the first and the last
+timeslice against the oracle (a subset of derivatives and momenta: the
+whole timeslice would take minutes in numpy), V^dag V = 1 on every
+timeslice and the mixed form against the FP64 form everywhere.
+
+``test_config_c_shape_properties`` (one timeslice) checks properties that
+hold at any size:
 
 * V^dag V = 1 (d = 0, p = 0) to complex64 rounding;
 * E_{rev d}[p] = (-1)^{|d|} E_d[-p]^dagger with every derivative computed
@@ -105,3 +114,58 @@ def test_config_c_shape_properties():
     err = blockwise(Em, E)
     say(f"  mixed vs fp64 {err:.2e}")
     assert err <= 1e-6
+
+
+def test_config_c_full_launch_against_oracle():
+    import torch
+
+    latt, Ne, nn = [32, 32, 32, 8], 200, 2
+    Lt = latt[3]
+    moms = S.momenta_by_norm(4)
+    D, P = ed.num_derivative(nn), len(moms)
+    idx = {m: i for i, m in enumerate(moms)}
+    dev = torch.device("cuda:0")
+    say("config C, 8 timeslices: inputs")
+    U = S.gauge_torch(latt, "haar", seed=41, device=dev)
+    V = S.eigvecs_torch(latt, Ne, seed=42, device=dev)
+
+    outs = {}
+    for mode in ("fp64", "mixed"):
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode=mode)
+        g.set_option(_native.ED_OPT_PROFILE, 1)
+        g.load_arrays(U, V)
+        say(f"config C, 8 timeslices: {mode}")
+        outs[mode] = g.calc_all(out=torch.empty((Lt, D, P, Ne, Ne), dtype=torch.complex128, device=dev))
+        torch.cuda.synchronize()
+        ms, launches = g.context.profile_read(_native.ED_KERNEL_CONTRACT)
+        say(f"  {launches} contraction launch(es), {ms:.1f} ms")
+        # all eight timeslices in one automatic launch, as the benchmark times it
+        assert launches == 1
+        del g
+    E64, Em = outs["fp64"], outs["mixed"]
+
+    eye = torch.eye(Ne, dtype=torch.complex128, device=dev)
+    vv = float((E64[:, 0, idx[(0, 0, 0)]] - eye).abs().max())
+    say(f"  V^dag V - 1 over 8 timeslices: {vv:.2e}")
+    assert vv < 1e-5
+    err = blockwise(Em, E64)
+    say(f"  mixed vs fp64 over 8 timeslices: {err:.2e}")
+    assert err <= 1e-6
+
+    # derivatives 5 (x then y) and 7 (y then x) are a Hermitian pair: one is
+    # contracted, the other written as its partner; 12 (z z) is palindromic
+    checks = [(0, [0, 5, 7, 12], [(0, 0, 0), (1, 0, 0), (1, 1, -1), (0, -2, 0)]),
+              (Lt - 1, [2, 9], [(-1, 0, 1), (0, 0, 2)])]
+    for t, dsel, msel in checks:
+        say(f"  oracle at t={t}: derivatives {dsel}, momenta {msel}")
+        Vt = V[t].cpu().numpy()
+        Ut = U[:, t].cpu().numpy()
+        ref = O.elemental_timeslice(Vt, Ut, latt, nn, msel, derivative_indices=dsel)
+        pidx = [idx[m] for m in msel]
+        for mode, tol in (("fp64", 1e-12), ("mixed", 1e-6)):
+            got = outs[mode][t][dsel][:, pidx].cpu().numpy()
+            d = np.abs(got - ref).reshape(len(dsel), len(msel), -1).max(-1)
+            r = np.abs(ref).reshape(len(dsel), len(msel), -1).max(-1)
+            e = float((d / r).max())
+            say(f"    {mode}: max blockwise {e:.2e}")
+            assert e <= tol

@@ -305,6 +305,42 @@ def test_load_from_ildg_and_qdp_files(tmp_path):
     assert err <= FP64_TOL
 
 
+def test_load_from_gauge_timeslice_and_binary_files(tmp_path):
+    """load(key) through the other two gauge containers of the reference:
+    QDP LazyDiskMapObj records per (t, mu) (GaugeFieldTimeSlice, preset.py:44-53;
+    read back in complex64, so the oracle gets the same rounded links) and a
+    headerless binary array of interleaved float64 (GaugeFieldBinary,
+    preset.py:162-170), with npy eigenvectors."""
+    from easydistillation_amd.filedata import write_binary, write_qdp_lazy
+
+    latt, Ne, nn = [4, 3, 5, 2], 6, 2
+    Lx, Ly, Lz, Lt = latt
+    V3 = Lx * Ly * Lz
+    moms = S.momenta_by_norm(2)
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=9)
+    full = np.empty((Lt, Lz, Ly, Lx, 4, 3, 3), np.complex128)
+    full[..., :3, :, :] = U.transpose(1, 2, 3, 4, 0, 5, 6)
+    full[..., 3, :, :] = np.eye(3)
+    per_t_mu = full.reshape(Lt, V3, 4, 3, 3).transpose(0, 2, 1, 3, 4)
+    write_qdp_lazy(str(tmp_path / "cfg.mod"), {(t, mu): per_t_mu[t, mu] for t in range(Lt) for mu in range(4)},
+                   latt, dtype=">c16")
+    inter = np.stack([full.real, full.imag], -1).reshape(Lt, V3, 4, 3, 3, 2)
+    write_binary(str(tmp_path / "cfg.dat"), inter, "<f8")
+    np.save(tmp_path / "cfg.npy", V)
+    eig = ed.EigenvectorNpy(str(tmp_path) + "/", ".npy", None, Ne)
+    cases = [
+        ("timeslice", ed.GaugeFieldTimeSlice(str(tmp_path) + "/", ".mod", [Lt, 4, V3, 3, 3]),
+         U.astype(np.complex64).astype(np.complex128)),
+        ("binary", ed.GaugeFieldBinary(str(tmp_path) + "/", ".dat", [Lt, V3, 4, 3, 3, 2]), U),
+    ]
+    for name, gauge, Uref in cases:
+        g = ed.ElementalGenerator(latt, gauge, eig, nn, moms)
+        g.load("cfg")
+        err = blockwise(g.calc_all(), O.elemental(V, Uref, latt, nn, moms))
+        say(f"{name} gauge container: err={err:.3e}")
+        assert err <= FP64_TOL
+
+
 def test_mixed_below_4096_sites_runs_fp64_kernels():
     """Below V3 = 4096 the mixed form's default is the FP64 kernels (bit-identical to mode
     "fp64"); an explicit ED_OPT_MIXED_STAGE2 selects the mixed kernels again."""

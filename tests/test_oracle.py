@@ -360,3 +360,15 @@ def test_displacement_gauge_invariance():
     a = O.displacement_timeslice(V, U, latt, 3, moms, round_v_to_c64=False)
     b = O.displacement_timeslice(Vg, Ug, latt, 3, moms, round_v_to_c64=False)
     assert nrm(b, a) < 1e-13
+
+
+def test_derivative_subset_equals_the_full_output():
+    """The subset option (used for spot checks at large shapes) returns exactly
+    the selected derivative blocks of the full computation, in the given order."""
+    latt = [3, 4, 2, 1]
+    U, V = S.make_inputs_np(latt, 5, "haar", seed=11)
+    moms = [(0, 0, 0), (1, 0, -1), (0, 1, 0)]
+    full = O.elemental_timeslice(V[0], U[:, 0], latt, 2, moms)
+    sel = [7, 0, 12, 5]
+    sub = O.elemental_timeslice(V[0], U[:, 0], latt, 2, moms, derivative_indices=sel)
+    assert np.array_equal(sub, full[sel])
