@@ -72,21 +72,62 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def blockwise_error(E, Eref):
-    """max over (d,p) of max|dE| / max|Eref| (torch tensors [Lt, D, P, Ne, Ne]), and the mean."""
+def blockwise_error(E, Eref, tables=False):
+    """Error of E against Eref (torch tensors [Lt, D, P, Ne, Ne]) per (d, p) block.
+
+    Returns (max, mean) over blocks of the normwise max error
+    max_{t,e,f}|dE| / max_{t,e,f}|Eref|; with ``tables`` also the [D][P]
+    tables of that error and of the Frobenius error ||dE||_F / ||Eref||_F.
+    """
     import torch
 
     nd, npm = E.shape[1], E.shape[2]
     dmax = torch.zeros((nd, npm), dtype=torch.float64, device=E.device)
     rmax = torch.zeros_like(dmax)
+    dfro = torch.zeros_like(dmax)
+    rfro = torch.zeros_like(dmax)
     for d in range(nd):  # one derivative at a time bounds the temporary memory
-        diff = (E[:, d] - Eref[:, d]).abs()
-        dmax[d] = diff.permute(1, 0, 2, 3).reshape(npm, -1).amax(-1)
-        rmax[d] = Eref[:, d].abs().permute(1, 0, 2, 3).reshape(npm, -1).amax(-1)
+        diff = (E[:, d] - Eref[:, d]).abs().permute(1, 0, 2, 3).reshape(npm, -1)
+        ref = Eref[:, d].abs().permute(1, 0, 2, 3).reshape(npm, -1)
+        dmax[d], rmax[d] = diff.amax(-1), ref.amax(-1)
+        dfro[d], rfro[d] = diff.square().sum(-1), ref.square().sum(-1)
     row = rmax.amax(1, keepdim=True)
     den = torch.where(rmax >= 1e-8 * row, rmax, row.expand_as(rmax))
     err = dmax / den
-    return float(err.max()), float(err.mean())
+    if not tables:
+        return float(err.max()), float(err.mean())
+    fro = (dfro / torch.where(rmax >= 1e-8 * row, rfro, rfro.amax(1, keepdim=True).expand_as(rfro))).sqrt()
+
+    def table(x):
+        return [[float(f"{v:.3g}") for v in r] for r in x.cpu().tolist()]
+
+    return float(err.max()), float(err.mean()), table(err), table(fro)
+
+
+def host_description():
+    """CPU model, thread count and the BLAS numpy runs on (threadpoolctl's view)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    blas = []
+    try:
+        from threadpoolctl import threadpool_info
+
+        blas = [f"{i.get('internal_api')} {i.get('version')} ({i.get('threading_layer', '')})"
+                for i in threadpool_info() if i.get("user_api") == "blas"]
+    except Exception:
+        pass
+    return model, cores, blas
 
 
 def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
@@ -100,10 +141,22 @@ def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
     """
     import numpy as np
 
+    from threadpoolctl import threadpool_limits
+
     from easydistillation_amd import synthetic as S
     from oracle.elemental_oracle import derivative_list
     from oracle.reference_port import ReferencePortGenerator
 
+    model, avail, _ = host_description()
+    # every host core this process may use (the box's CPU share), set explicitly for the BLAS
+    threads = int(os.environ.get("OMP_NUM_THREADS") or avail)
+    with threadpool_limits(limits=threads):
+        return _cpu_sample(latt, Ne, num_nabla, moms, seconds_target, threads, model, np, S, derivative_list,
+                           ReferencePortGenerator)
+
+
+def _cpu_sample(latt, Ne, num_nabla, moms, seconds_target, threads, model, np, S, derivative_list,
+                ReferencePortGenerator):
     Lx, Ly, Lz, _ = latt
     one = [Lx, Ly, Lz, 1]
     rng = np.random.default_rng(0)
@@ -139,17 +192,20 @@ def cpu_baseline(latt, Ne, num_nabla, moms, seconds_target=20.0):
             break
     per_t = n_hops * (t_hop / n_h) + n_splits * len(moms) * (t_con / n_c)
     elementals_t = len(moms) * len(dl) * Ne * Ne
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    _, _, blas = host_description()
     return {
         "value": elementals_t / per_t,
         "unit": "elementals/s",
         "cores": threads,
         "kind": "port",
+        "cpu": model,
+        "blas": blas,
+        "numpy": np.__version__,
+        "threads_set": f"threadpoolctl limit {threads} (BLAS and OpenMP pools)",
         "sample": (
             f"timeslice of {Lx}x{Ly}x{Lz}, Ne={Ne}: {n_h} hop applications ({t_hop / n_h:.2f} s each) and "
             f"{n_c} split contractions ({t_con / n_c:.2f} s each) of the reference loop nest, extrapolated "
-            f"to its {n_hops} hops + {n_splits}x{len(moms)} contractions per timeslice ({per_t:.0f} s/t); "
-            f"numpy {np.__version__}, {threads} threads, {platform.processor() or platform.machine()}"
+            f"to its {n_hops} hops + {n_splits}x{len(moms)} contractions per timeslice ({per_t:.0f} s/t)"
         ),
     }
 
@@ -391,6 +447,8 @@ def main():
     ap.add_argument("--tbatch", type=int, default=0, help="timeslices per contraction launch (0: the library's choice)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--ref-loop-t", type=int, default=16,
+                    help="timeslices of the reference-style 'for t: data[t] = calc(t)' loop to time (0: skip)")
     ap.add_argument(
         "--calibrate-copy",
         type=int,
@@ -511,7 +569,7 @@ def main():
 
     err = None
     if "mixed" in outs and "fp64" in outs:
-        err = blockwise_error(outs["mixed"], outs["fp64"])
+        err = blockwise_error(outs["mixed"], outs["fp64"], tables=True)
         log(f"mixed vs fp64 blockwise error: max {err[0]:.3e} mean {err[1]:.3e}")
     # V^dag V = 1 sanity check on the reference-numerics form
     chk = outs.get("fp64", outs.get("mixed"))
@@ -521,6 +579,39 @@ def main():
 
     head = "mixed" if "mixed" in results else forms[0]
     r = results[head]
+
+    ref_loop = None
+    if args.ref_loop_t > 0:
+        # the reference's driver: data[t] = calc(t) into a preallocated host array
+        # (tests/test_elemental.py:37-45), here over the first ref_loop_t timeslices
+        import numpy as np
+
+        nt = min(Lt, args.ref_loop_t)
+        g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode=head)
+        g.load_arrays(U, V)
+        data = np.zeros((nt, D, P, Ne, Ne), dtype=np.complex128)
+        g.calc(0)  # warm-up: kernels, page-locked staging buffer
+        g.load_arrays(U, V)  # drops the computed window
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for t in range(nt):
+            data[t] = g.calc(t)
+        dt_loop = (time.perf_counter() - t1) / nt
+        chk = float(np.abs(data[0, 0, moms.index((0, 0, 0))] - np.eye(Ne)).max())
+        ref_loop = {
+            "form": head,
+            "timeslices": nt,
+            "ms_per_timeslice": dt_loop * 1e3,
+            "elementals_per_s": P * D * Ne * Ne / dt_loop,
+            "identity_check_max_abs": chk,
+            "note": (
+                f"'for t: data[t] = calc(t)' into a preallocated numpy array; calc(t) computes aligned "
+                f"windows of {g.WINDOW} timeslices per launch into HBM and copies each timeslice out through "
+                f"a page-locked buffer ({D * P * Ne * Ne * 16 / 1e6:.0f} MB per timeslice, PCIe included)"
+            ),
+        }
+        log(f"reference-style loop: {json.dumps(ref_loop)}")
+        del g, data
     # the measured bytes hold for the library's own timeslice batch (8 per launch at this workload)
     traffic_key = (args.workload, head, args.mixed_ws) if args.tbatch == 0 else None
     line = {
@@ -534,7 +625,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if head == "mixed" else "fp64",
+        "dtype": "fp32+fp16x3" if head == "mixed" and r["mixed_kernel"] in (2, 3, 4, 5) else (
+            "fp32" if head == "mixed" else "fp64"),
+        "precision": (
+            {
+                "stencil": "FP64 arithmetic, complex64 storage",
+                "split_sum": "FP32 MFMA",
+                "phase_contraction": "FP16 MFMA on a three-term hi/lo split of phase and Q, FP32 partial sums per 32 sites",
+                "reduction_and_phase_table": "FP64",
+            }
+            if head == "mixed" and r["mixed_kernel"] in (2, 3, 4, 5)
+            else None
+        ),
         "data": "synthetic (seeded Haar SU(3) links, orthonormal complex64 eigenvectors)",
         "config": {
             "workload": f"{Lx}^3x{Lt}" if Lx == Ly == Lz else f"{Lx}x{Ly}x{Lz}x{Lt}",
@@ -565,7 +667,15 @@ def main():
             "algorithmic_equivalent_tflops": r["contract_tflops_algorithmic"],
         },
         "forms": results,
-        "mixed_vs_fp64_error": None if err is None else {"max": err[0], "mean": err[1], "metric": "per (d,p) max|dE|/max|E_fp64|"},
+        "mixed_vs_fp64_error": None if err is None else {
+            "max": err[0],
+            "mean": err[1],
+            "metric": "per (d,p) max|dE|/max|E_fp64|",
+            "per_dp_normwise_max": err[2],
+            "per_dp_frobenius": err[3],
+            "tables": (f"[d][p]: d in derivative index order (derivative.py:23-33), p over the {P} momenta with "
+                       f"|n|^2 <= {n2} sorted by |n|^2, then lexicographically"),
+        },
         "identity_check_max_abs": eye_err,
     }
     if "fp64" in results and "mixed" in results:
@@ -578,6 +688,8 @@ def main():
             "unit": "GB/s",
             "frac": results["fp64"]["stencil_gbs"] / HBM_PEAK_GBS,
         }
+    if ref_loop is not None:
+        line["reference_loop"] = ref_loop
     if not args.no_cpu_baseline:
         log("cpu baseline sample ...")
         line["cpu_baseline"] = cpu_baseline(latt, Ne, num_nabla, moms, args.cpu_seconds)

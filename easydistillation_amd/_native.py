@@ -22,6 +22,7 @@ ED_OPT_HERMITIAN, ED_OPT_TBATCH, ED_OPT_WS_MB, ED_OPT_PROFILE, ED_OPT_MIXED_STAG
 ED_OPT_EXACT_V = 6
 ED_OPT_MIXED_WS = 7
 ED_OPT_WIDE_WS = 8
+ED_OPT_ORTHO_GATE = 9
 ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE, ED_KERNEL_ADJOINT = 0, 1, 2, 3
 
 # Every exported symbol and its ctypes signature (restype, argtypes).
@@ -42,6 +43,7 @@ SIGNATURES = {
     "ed_set_weights": (_I, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "ed_compute": (_I, [_VP, _I, _I, _I, _VP, _I]),
     "ed_synchronize": (_I, [_VP]),
+    "ed_orthonormality": (_I, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "ed_set_option": (_I, [_VP, _I, ctypes.c_int64]),
     "ed_profile_read": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "ed_profile_reset": (_I, [_VP]),
@@ -166,6 +168,12 @@ class Context:
 
     def synchronize(self) -> None:
         self._chk("ed_synchronize", self.lib.ed_synchronize(self.h))
+
+    def orthonormality(self) -> float:
+        """FP64 max|V^dag V - 1| of the eigenvectors last set."""
+        v = ctypes.c_double()
+        self._chk("ed_orthonormality", self.lib.ed_orthonormality(self.h, ctypes.byref(v)))
+        return v.value
 
     def profile_read(self, kernel: int):
         ms = ctypes.c_double()

@@ -159,6 +159,8 @@ struct ed_context {
   int mixed_ws = 1;
   int wide_ws = 0;  // 64-bit workspace addresses in the warp-specialized kernel below 2^31 bytes too
   float vmax = 0.f;        // largest |re|, |im| of the stored eigenvectors
+  double vv_dev = -1.0;    // FP64 max|V^dag V - 1| over timeslices of the stored eigenvectors
+  int ortho_gate = 0;      // n > 0: ed_set_eigvecs refuses vv_dev > 10^-n
   int wsh = 0;             // scale exponent of the FP16 stage-1 workspace
   int W_elem = 0;          // bytes per complex element of the data now in W (0: none)
   uint64_t gen = 0;        // bumped whenever buffers or plan a captured launch refers to change
@@ -539,6 +541,53 @@ __global__ void absmax_c64_kernel(const float2* __restrict__ in, size_t n, unsig
   v = ed::wave_max_nonneg(v);
   if ((threadIdx.x & 63) == 0) atomicMax(m, __float_as_uint(v));
 }
+// FP64 max |(V_t^dag V_t)[e][f] - delta_ef| over one 32 x 32 tile (e, f) of
+// timeslice t0 + blockIdx.z; each row of V_t holds K = 3 V3 complex64 values.
+// Products of complex64 values are exact in FP64 and the sums run in FP64.
+// The tile maximum is maxed into *dev as the bit pattern of a non-negative
+// double (unsigned order is double order); *dev starts at 0.
+__global__ __launch_bounds__(256) void gram_dev_kernel(const float2* __restrict__ V, int Ne, size_t K, int t0,
+                                                      unsigned long long* __restrict__ dev) {
+  constexpr int KC = 32;
+  __shared__ double2 A[32][KC + 1], B[32][KC + 1];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int e0 = blockIdx.x * 32, f0 = blockIdx.y * 32;
+  const float2* Vt = V + (size_t)(t0 + blockIdx.z) * Ne * K;
+  double re[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, im[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (size_t k0 = 0; k0 < K; k0 += KC) {
+    for (int i = tid; i < 32 * KC; i += 256) {
+      const int r = i / KC, k = i % KC;
+      const size_t kk = k0 + k;
+      float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+      if (kk < K) {
+        if (e0 + r < Ne) a = Vt[(size_t)(e0 + r) * K + kk];
+        if (f0 + r < Ne) b = Vt[(size_t)(f0 + r) * K + kk];
+      }
+      A[r][k] = make_double2(a.x, a.y);
+      B[r][k] = make_double2(b.x, b.y);
+    }
+    __syncthreads();
+    for (int k = 0; k < KC; ++k) {
+      const double2 a[2] = {A[ty][k], A[ty + 16][k]};
+      const double2 b[2] = {B[tx][k], B[tx + 16][k]};
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {  // conj(a) b
+          re[i][j] += a[i].x * b[j].x + a[i].y * b[j].y;
+          im[i][j] += a[i].x * b[j].y - a[i].y * b[j].x;
+        }
+    }
+    __syncthreads();
+  }
+  double d = 0.0;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      const int e = e0 + ty + 16 * i, f = f0 + tx + 16 * j;
+      if (e < Ne && f < Ne) d = fmax(d, hypot(re[i][j] - (e == f ? 1.0 : 0.0), im[i][j]));
+    }
+  for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_xor(d, off));
+  if ((tid & 63) == 0) atomicMax(dev, (unsigned long long)__double_as_longlong(d));
+}
+
 __global__ void c64_to_c128_kernel(const float2* __restrict__ in, double2* __restrict__ out, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     out[i] = make_double2((double)in[i].x, (double)in[i].y);
@@ -560,9 +609,14 @@ bool ws_wide(const ed_context* c) { return !ws_fits32(c) || c->wide_ws; }
 // the mixed form takes the FP16-MFMA phase contraction when chosen and its
 // kernel can address the workspace (every FP16 stage-2 kernel below 2^31
 // bytes per timeslice, the warp-specialized one at any size)
+// The FP16 phase table of one momentum block is read through a buffer
+// resource with 32-bit byte offsets: V3P x 128 bytes must stay below 2^31
+// (V3 below 2^24 sites); larger lattices take the FP32 phase contraction.
+bool ph16_fits32(const ed_context* c) { return (size_t)c->V3P * 128 < ((size_t)1 << 31); }
+
 bool use_f16_stage2(const ed_context* c) {
   const bool chosen = c->mixed_stage2 == 1 || (c->mixed_stage2 < 0 && c->V3 >= 4096);
-  return chosen && (ws_fits32(c) || use_ws_kernel(c));
+  return chosen && ph16_fits32(c) && (ws_fits32(c) || use_ws_kernel(c));
 }
 
 // The mixed form on lattices with V3 < 4096 and the default phase
@@ -804,7 +858,13 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
   if (!o_dev) per_t += ((gV ? vt : 0) + (gU ? ut : 0)) * sizeof(double2);
   int TB = c->tbatch_opt;
   if (TB <= 0) {
-    const size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
+    // at most the reverse-pass buffers already held (reused) plus three
+    // quarters of the free device memory, as for the forward batch
+    const size_t held =
+        (c->bW.n + c->bgW.n + c->bA.n + c->bG.n + c->bOutV.n + c->bOutU.n) * sizeof(double2);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = std::min(budget, held + free_b / 4 * 3);
     TB = (int)std::max<size_t>(1, budget / per_t);
   }
   TB = std::min(TB, nt);
@@ -954,6 +1014,11 @@ struct ed_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // ordering against the member contexts' streams: one event per item
+  // (recorded on its context's stream before a run) and one recorded on the
+  // batch stream after it
+  std::vector<hipEvent_t> ctx_ev;
+  hipEvent_t done_ev = nullptr;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -966,6 +1031,8 @@ struct ed_batch {
     drop_graph();
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto e : ctx_ev) (void)hipEventDestroy(e);
+    if (done_ev) (void)hipEventDestroy(done_ev);
   }
 };
 
@@ -1109,6 +1176,32 @@ void batch_enqueue(ed_batch* b) {
   }
 }
 
+// The batch stream waits for everything enqueued so far on each member
+// context's stream (ed_compute, ed_set_eigvecs from device memory,
+// ed_stout_smear write the workspace, V and the links the batch reads).
+void batch_wait_members(ed_batch* b) {
+  if (b->ctx_ev.size() < b->items.size()) {
+    for (size_t i = b->ctx_ev.size(); i < b->items.size(); ++i) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->ctx_ev.push_back(e);
+    }
+  }
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    HIPCHK(hipEventRecord(b->ctx_ev[i], b->items[i].ctx->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ctx_ev[i], 0));
+  }
+}
+
+// Each member context's stream waits for the run just enqueued on the batch
+// stream, so later calls on a context see the batch's outputs and do not
+// overwrite a workspace the batch still reads.
+void batch_release_members(ed_batch* b) {
+  if (!b->done_ev) HIPCHK(hipEventCreateWithFlags(&b->done_ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(b->done_ev, b->stream));
+  for (auto& it : b->items) HIPCHK(hipStreamWaitEvent(it.ctx->stream, b->done_ev, 0));
+}
+
 void batch_run(ed_batch* b) {
   require(!b->items.empty(), ED_ERR_STATE, "the batch is empty");
   HIPCHK(hipSetDevice(b->device));
@@ -1119,7 +1212,9 @@ void batch_run(ed_batch* b) {
   if (!b->use_graph) {
     b->drop_graph();
     batch_prepare(b);
+    batch_wait_members(b);
     batch_enqueue(b);
+    batch_release_members(b);
     return;
   }
   // a fresh plan may reallocate buffers; recapture when anything moved
@@ -1139,7 +1234,9 @@ void batch_run(ed_batch* b) {
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->fingerprint = fp;
   }
+  batch_wait_members(b);
   HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  batch_release_members(b);
 }
 
 double batch_time(ed_batch* b, int reps) {
@@ -1313,6 +1410,27 @@ int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device) {
       if (!std::isfinite(f)) fail(ED_ERR_ARG, "eigenvectors hold a non-finite value");
       ctx->vmax = f;
     }
+    {
+      // FP64 orthonormality of the stored eigenvectors, every timeslice
+      DevBuf<unsigned long long> dv;
+      dv.ensure(1);
+      HIPCHK(hipMemsetAsync(dv.p, 0, sizeof(unsigned long long), ctx->stream));
+      const size_t K = (size_t)3 * ctx->V3;
+      const unsigned nb = (unsigned)((ctx->Ne + 31) / 32);
+      for (int t0 = 0; t0 < ctx->Lt; t0 += 1024) {
+        const unsigned nz = (unsigned)std::min(1024, ctx->Lt - t0);
+        gram_dev_kernel<<<dim3(nb, nb, nz), 256, 0, ctx->stream>>>(ctx->V.p, ctx->Ne, K, t0, dv.p);
+        HIPCHK(hipGetLastError());
+      }
+      unsigned long long h = 0;
+      HIPCHK(hipMemcpyAsync(&h, dv.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      std::memcpy(&ctx->vv_dev, &h, sizeof(double));
+      ctx->eig_set = false;
+      if (ctx->ortho_gate > 0 && !(ctx->vv_dev <= std::pow(10.0, -ctx->ortho_gate)))
+        fail(ED_ERR_ARG, "eigenvectors are not orthonormal: max|V^dag V - 1| = %.3e > 1e-%d", ctx->vv_dev,
+             ctx->ortho_gate);
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->eig_set = true;
   });
@@ -1418,6 +1536,10 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
       case ED_OPT_WIDE_WS:
         ctx->wide_ws = value ? 1 : 0;
         break;
+      case ED_OPT_ORTHO_GATE:
+        if (value < 0 || value > 15) fail(ED_ERR_ARG, "orthonormality gate must be in [0, 15]");
+        ctx->ortho_gate = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown option %d", option);
     }
@@ -1507,6 +1629,15 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
       case ED_Q_MIXED_KERNEL: *value = mixed_kernel_id(ctx); break;
       default: fail(ED_ERR_ARG, "unknown query %d", what);
     }
+  });
+}
+
+int ed_orthonormality(ed_context* ctx, double* max_dev) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (!max_dev) fail(ED_ERR_ARG, "max_dev is null");
+    require(ctx->vv_dev >= 0.0, ED_ERR_STATE, "ed_set_eigvecs has not measured any eigenvectors yet");
+    *max_dev = ctx->vv_dev;
   });
 }
 

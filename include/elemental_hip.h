@@ -46,8 +46,11 @@ enum ed_dtype { ED_C128 = 0, ED_C64 = 1 };
 
 enum ed_mode {
   ED_MODE_FP64 = 0,  /* complex128 stencil and contraction (reference numerics)          */
-  ED_MODE_MIXED = 1  /* complex64 stencil, FP32-MFMA contraction, FP64 phase table and  */
-                     /* FP64 cross-chunk spatial reduction                              */
+  ED_MODE_MIXED = 1  /* hops in FP64 arithmetic stored as complex64; contraction with   */
+                     /* the split sum on FP32 MFMA and the momentum phase on FP16 MFMA   */
+                     /* (three-term hi/lo split, FP32 partial sums per 32 sites) from    */
+                     /* V3 = 4096 sites on, FP64 kernels below; FP64 phase table, zero-  */
+                     /* momentum sum and cross-chunk spatial reduction                   */
 };
 
 /* ed_create / ed_destroy: one context per device and host thread.
@@ -75,8 +78,15 @@ int ed_set_gauge(ed_context* ctx, const void* U, int dtype, int on_device);
 /* Eigenvectors V[Lt][Ne][Lz][Ly][Lx][3] (eigenvector.load(key)[t, e],
  * elemental.py:297-298).  Stored as complex64: complex128 input is rounded
  * to complex64 exactly as the reference's c8 staging buffer does
- * (elemental.py:55). */
+ * (elemental.py:55).  The call also measures, in FP64, the orthonormality
+ * of what it stored, max over t, e, f of |(V_t^dag V_t)[e][f] - delta_ef|
+ * (read by ed_orthonormality; gated by ED_OPT_ORTHO_GATE). */
 int ed_set_eigvecs(ed_context* ctx, const void* V, int dtype, int on_device);
+
+/* FP64 max|V^dag V - 1| of the eigenvectors the last ed_set_eigvecs stored
+ * (no counterpart in the reference, which reads V without a check;
+ * elemental.py:297-298).  ED_ERR_STATE before any ed_set_eigvecs. */
+int ed_orthonormality(ed_context* ctx, double* max_dev);
 
 /* Momentum list n[P][3] = (nx, ny, nz); output index p follows this order
  * and Phi_n(x) = exp(+2 pi i (nx x/Lx + ny y/Ly + nz z/Lz))
@@ -147,7 +157,11 @@ int ed_synchronize(ed_context* ctx);
  *                         64-bit-address form (derivatives of more than four split terms then
  *                         take the FP32 phase contraction).
  *   ED_OPT_WIDE_WS        1: kind 1 uses its 64-bit-address form below 2^31 bytes too (same
- *                         results bit for bit; for testing). */
+ *                         results bit for bit; for testing).
+ *   ED_OPT_ORTHO_GATE     n in [1, 15]: ed_set_eigvecs fails with ED_ERR_ARG (and the context
+ *                         holds no eigenvectors) when the FP64 max|V^dag V - 1| of the stored
+ *                         eigenvectors exceeds 10^-n; 0 (default): measured and reported by
+ *                         ed_orthonormality only. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
@@ -156,7 +170,8 @@ enum ed_option {
   ED_OPT_MIXED_STAGE2 = 5,
   ED_OPT_EXACT_V = 6,
   ED_OPT_MIXED_WS = 7,
-  ED_OPT_WIDE_WS = 8
+  ED_OPT_WIDE_WS = 8,
+  ED_OPT_ORTHO_GATE = 9
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 
@@ -263,6 +278,12 @@ int ed_get_gauge(ed_context* ctx, void* U, int on_device);
  *   ed_batch_add: out_device is a device array [Lt][D][P][Ne][Ne] complex128
  *                 receiving every timeslice of ctx in mode (FP64 or MIXED).
  *   ed_batch_run: enqueue on the batch's own stream; ed_batch_synchronize waits.
+ *                 The run first waits (stream-ordered, no host wait) for
+ *                 work already enqueued on each member context's stream, and
+ *                 each member's stream then waits for the run, so calls on a
+ *                 context before and after a run are ordered with it.
+ *                 The batch keeps the member context pointers: destroy the
+ *                 batch before any of its contexts.
  *   ed_batch_time: mean duration (HIP events) of reps runs after one warm-up.
  *   ed_batch_tune: autotuner -- times every launch shape (persistent grid of
  *                 1, 2 or 4 workgroups per CU, or one workgroup per work item)

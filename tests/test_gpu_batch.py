@@ -135,3 +135,33 @@ def test_batch_rejects_duplicates_and_empty():
     nb.add(gens[0].context, _native.ED_MODE_FP64, out.data_ptr())
     with pytest.raises(RuntimeError):
         nb.add(gens[0].context, _native.ED_MODE_FP64, out.data_ptr())
+
+
+def test_batch_is_ordered_with_member_streams():
+    """Calls through the C ABI that return before their work is done (device
+    output, device eigenvectors) are ordered with a batch run: the batch
+    waits for each member context's stream and the streams wait for the
+    batch, with no host synchronization in between."""
+    gens, inputs = make_generators(SPECS[2:], seed=40)
+    singles = [g.calc_all() for g in gens]
+    b = ElementalBatch(gens)
+    b.run()
+    torch.cuda.synchronize()
+    # enqueue new eigenvectors from device memory and an ed_compute into a device buffer on
+    # each context's own stream, then run the batch at once
+    outs_dev = []
+    for g, (U, V) in zip(gens, inputs):
+        Vd = torch.from_numpy(V).to("cuda:0")
+        torch.cuda.synchronize()
+        g.context.set_eigvecs(Vd.data_ptr(), _native.ED_C64, True)
+        o = torch.empty_like(b.outputs[gens.index(g)])
+        g.context.compute(0, g.latt_size[3], _native.ED_MODE_FP64 if g.mode == "fp64" else _native.ED_MODE_MIXED,
+                          o.data_ptr(), True)
+        outs_dev.append((o, Vd))
+    b.batch.run()
+    b.batch.synchronize()
+    for g in gens:
+        g.context.synchronize()
+    for g, s, o, (od, _) in zip(gens, singles, b.outputs, outs_dev):
+        assert np.array_equal(o.cpu().numpy(), s)
+        assert np.array_equal(od.cpu().numpy(), s)

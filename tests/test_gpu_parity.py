@@ -149,6 +149,32 @@ def test_calc_paths_agree():
     assert blockwise(batched, Eref) <= FP64_TOL
 
 
+def test_calc_windows_any_order_and_reload():
+    """calc(t) computes aligned windows of timeslices in one launch; any call
+    order, a partial last window, a mode switch and new inputs all return the
+    right timeslice."""
+    latt, Ne, nn = [4, 5, 3, 11], 8, 1
+    moms = S.momenta_by_norm(1)
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=15)
+    U2, V2 = S.make_inputs_np(latt, Ne, "smooth", seed=16)
+    ref = O.elemental(V, U, latt, nn, moms)
+    ref2 = O.elemental(V2, U2, latt, nn, moms)
+    g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne)
+    g.WINDOW = 4  # windows [0, 4), [4, 8), [8, 11)
+    g.load_arrays(U, V)
+    for t in [7, 0, 10, 3, 8, 4, 9, 10]:
+        E = g.calc(t)
+        assert E.shape == ref.shape[1:] and E.flags.c_contiguous
+        assert blockwise(E[None], ref[t][None]) <= FP64_TOL, t
+    a, b = g.calc(5), g.calc(5)
+    a[:] = 0  # fresh arrays: one does not alias the other
+    assert np.array_equal(b, g.calc(5)) and not np.array_equal(a, b)
+    assert blockwise(g.calc(2, mode="mixed")[None], ref[2][None]) <= MIXED_TOL
+    g.load_arrays(U2, V2)
+    for t in [6, 1]:
+        assert blockwise(g.calc(t)[None], ref2[t][None]) <= FP64_TOL, t
+
+
 def test_load_from_containers(tmp_path):
     latt = [4, 4, 4, 3]
     Lx, Ly, Lz, Lt = latt
@@ -378,3 +404,37 @@ def test_mixed_kernel_query_follows_the_routing():
         got = g.context.query(_native.ED_Q_MIXED_KERNEL)
         say(f"mixed kernel for {latt} {opts}: {got}")
         assert got == want
+
+
+def test_orthonormality_gate():
+    """ed_set_eigvecs measures max|V^dag V - 1| in FP64 on what it stored (the
+    complex64-rounded vectors); ED_OPT_ORTHO_GATE turns it into a refusal."""
+    latt, Ne = [6, 5, 4, 3], 9
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=12)
+    g = ed.ElementalGenerator(latt, None, None, 1, [(0, 0, 0)], Ne=Ne)
+    g.load_arrays(U, V)
+    Vc = V.astype(np.complex64).astype(np.complex128).reshape(latt[3], Ne, -1)
+    want = max(np.abs(Vc[t].conj() @ Vc[t].T - np.eye(Ne)).max() for t in range(latt[3]))
+    got = g.context.orthonormality()
+    say(f"orthonormal V: device {got:.3e}, host {want:.3e}")
+    # FP64 sums in another order: a few ulps of the unit diagonal
+    assert got < 1e-6 and abs(got - want) <= 1e-14
+    # one eigenvector of the last timeslice scaled by 1.01: |V^dag V - 1| = 0.0201 there
+    Vb = V.copy()
+    Vb[2, 4] *= 1.01
+    Vbc = Vb.astype(np.complex64).astype(np.complex128).reshape(latt[3], Ne, -1)
+    want = max(np.abs(Vbc[t].conj() @ Vbc[t].T - np.eye(Ne)).max() for t in range(latt[3]))
+    g.load_arrays(U, Vb)
+    got = g.context.orthonormality()
+    say(f"scaled eigenvector: device {got:.6e}, host {want:.6e}")
+    assert abs(got - want) <= 1e-12 and abs(got - 0.0201) < 1e-5
+    g.set_option(_native.ED_OPT_ORTHO_GATE, 6)
+    with pytest.raises(_native.NativeError) as e:
+        g.load_arrays(U, Vb)
+    assert e.value.code == _native.ED_ERR_ARG
+    with pytest.raises(_native.NativeError) as e:
+        g.calc_all()
+    assert e.value.code == _native.ED_ERR_STATE
+    g.load_arrays(U, V)  # orthonormal again: accepted
+    E = g.calc_all()
+    assert np.abs(E[:, 0, 0] - np.eye(Ne)).max() < 1e-6
