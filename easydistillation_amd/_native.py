@@ -42,6 +42,7 @@ SIGNATURES = {
     "ed_set_displacement": (_I, [_VP, _I]),
     "ed_set_weights": (_I, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "ed_compute": (_I, [_VP, _I, _I, _I, _VP, _I]),
+    "ed_compute_noalloc": (_I, [_VP, _I, _I, _I, _VP]),
     "ed_synchronize": (_I, [_VP]),
     "ed_orthonormality": (_I, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "ed_set_option": (_I, [_VP, _I, ctypes.c_int64]),
@@ -165,6 +166,10 @@ class Context:
 
     def compute(self, t0: int, nt: int, mode: int, out_ptr: int, out_on_device: bool) -> None:
         self._chk("ed_compute", self.lib.ed_compute(self.h, t0, nt, mode, _VP(out_ptr), int(out_on_device)))
+
+    def compute_noalloc(self, t0: int, nt: int, mode: int, out_ptr: int) -> None:
+        """ed_compute_noalloc: device output, fixed workspace, no allocation (enqueues only)."""
+        self._chk("ed_compute_noalloc", self.lib.ed_compute_noalloc(self.h, t0, nt, mode, _VP(out_ptr)))
 
     def synchronize(self) -> None:
         self._chk("ed_synchronize", self.lib.ed_synchronize(self.h))

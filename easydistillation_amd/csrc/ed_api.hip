@@ -9,6 +9,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,19 @@ struct EdError {
   va_end(ap);
   throw EdError{code, buf};
 }
+
+// ---------------------------------------------------------------------------
+// Fixed workspace of ed_compute_noalloc.  Each device that loads this library
+// holds its own copy until the process ends.  In host code the name
+// `scratch` is only a stand-in for the device symbol: its device address is
+// resolved once per device (resolve_scratch, called from ed_create with the
+// device current) and only that address is used afterwards.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[ED_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
 
 // derivative(n) of lattice/insertion/derivative.py:23-33: length-k tuples
 // occupy [(3^k-1)/2, (3^(k+1)-1)/2); digits most significant first.
@@ -755,14 +769,44 @@ ContractArgs contract_args(const ed_context* c, int nb, int t_out0) {
   return a;
 }
 
+__global__ void zero_fill_kernel(uint4* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Elementals of timeslices t0 .. t0+nt-1 into out_dev (device) or out_host.
+// With fixed != nullptr (ed_compute_noalloc) the workspace is the caller's
+// fixed buffer of fixed_bytes (device address): nothing is allocated, no
+// event is recorded and the host never waits on the stream; the used part of
+// the buffer is cleared by a fill kernel first, because padded rows and sites
+// must read zero and the buffer keeps whatever the previous call left.
 template <typename T>
-void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host) {
+void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host, char* fixed = nullptr,
+                 size_t fixed_bytes = 0) {
   const T* ph = nullptr;
   if (sizeof(T) == 8) ph = (const T*)(void*)c->ph64.p;
   else ph = (const T*)(void*)c->ph32.p;
   const size_t phn = (size_t)c->npb * 16 * c->V3P;
-  const int TB = auto_tbatch<T>(c, nt);
-  ed::c_t<T>* W = ensure_workspace<T>(c, TB);
+  const bool noalloc = fixed != nullptr;
+  int TB;
+  ed::c_t<T>* W;
+  if (noalloc) {
+    const size_t per_t = per_t_ws_bytes<T>(c);
+    if (per_t > fixed_bytes)
+      fail(ED_ERR_UNSUPPORTED,
+           "one timeslice needs %zu bytes of workspace; the no-allocation entry point holds at most %zu "
+           "(use ed_compute)", per_t, fixed_bytes);
+    TB = std::min<int>(nt, (int)std::min<size_t>(fixed_bytes / per_t, (size_t)nt));
+    if (c->tbatch_opt > 0) TB = std::min(TB, c->tbatch_opt);
+    W = (ed::c_t<T>*)(void*)fixed;
+    const size_t n16 = per_t * TB / sizeof(uint4);
+    zero_fill_kernel<<<(unsigned)std::min<size_t>(4096, (n16 + 255) / 256), 256, 0, c->stream>>>((uint4*)(void*)fixed,
+                                                                                                   n16);
+    HIPCHK(hipGetLastError());
+  } else {
+    TB = auto_tbatch<T>(c, nt);
+    W = ensure_workspace<T>(c, TB);
+  }
   const size_t out_t = (size_t)c->D * (c->moms.size() / 3) * c->Ne * c->Ne;
 
   double2* out = out_dev;
@@ -770,7 +814,14 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     c->stage.ensure(out_t * TB);
     out = c->stage.p;
   }
-  auto launch = [c](int kind, auto&& f) { c->launch(kind, f); };
+  auto launch = [c, noalloc](int kind, auto&& f) {
+    if (noalloc) {
+      f();
+      HIPCHK(hipGetLastError());
+    } else {
+      c->launch(kind, f);
+    }
+  };
 
   for (int tb0 = t0; tb0 < t0 + nt; tb0 += TB) {
     const int nb = std::min(TB, t0 + nt - tb0);
@@ -782,7 +833,7 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
       c->wsh = h16_shift(c);
       enqueue_stencil<ed::hsplit>(c, Wh, tb0, nb, c->stream, launch);
       a.g.wsh = c->wsh;
-      c->launch(ED_KERNEL_CONTRACT, [&] {
+      launch(ED_KERNEL_CONTRACT, [&] {
         if (h16_groups(c) <= 1)
           ed::contract_mixed16_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wh, c->ph16.p, out, a);
         else
@@ -792,7 +843,7 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
     enqueue_stencil<T>(c, W, tb0, nb, c->stream, launch);
     if (sizeof(T) == 4 && use_f16_stage2(c)) {
       const float2* Wf = (const float2*)(const void*)W;
-      c->launch(ED_KERNEL_CONTRACT, [&] {
+      launch(ED_KERNEL_CONTRACT, [&] {
         if (use_ws_kernel(c)) {
           const bool wide = ws_wide(c);
           auto ws = [&](auto nks) {
@@ -820,7 +871,7 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
           fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", 4 * c->nks_max / 3);
       });
     } else
-    c->launch(ED_KERNEL_CONTRACT, [&] {
+    launch(ED_KERNEL_CONTRACT, [&] {
       if (c->nks_max <= 1)
         ed::contract_kernel<T, 1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(W, ph, ph + phn, out, a);
       else if (c->nks_max <= 2)
@@ -839,7 +890,7 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
       HIPCHK(hipStreamSynchronize(c->stream));
     }
   }
-  if (c->profile) {
+  if (c->profile && !noalloc) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->drain_events();
   }
@@ -961,6 +1012,18 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
     HIPCHK(hipStreamSynchronize(c->stream));
     c->drain_events();
   }
+}
+
+// Device address of this device's `scratch`, resolved once per device with
+// the device current (the caller has set it).
+void resolve_scratch(int device) {
+  if (device >= kMaxDevices) fail(ED_ERR_UNSUPPORTED, "device %d: at most %d devices", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) fail(ED_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch workspace");
+  g_scratch_ptr[device] = (char*)p;
 }
 
 int guard(ed_context* ctx, const std::function<void()>& body) {
@@ -1274,6 +1337,7 @@ int ed_create(int device, ed_context** ctx) {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
     c->stream = c->own;
+    resolve_scratch(device);
   });
   if (rc != ED_OK) {
     delete c;
@@ -1560,6 +1624,27 @@ int ed_compute(ed_context* ctx, int t0, int nt, int mode, void* out, int out_on_
       run_batches<double>(ctx, t0, nt, out_on_device ? (double2*)out : nullptr, out_on_device ? nullptr : out);
     else
       run_batches<float>(ctx, t0, nt, out_on_device ? (double2*)out : nullptr, out_on_device ? nullptr : out);
+  });
+}
+
+int ed_compute_noalloc(ed_context* ctx, int t0, int nt, int mode, void* out_device) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set && ctx->gauge_set && ctx->eig_set && ctx->mom_set, ED_ERR_STATE,
+            "lattice, gauge, eigenvectors and momenta must be set before ed_compute_noalloc");
+    if (!out_device) fail(ED_ERR_ARG, "out_device is null");
+    if (t0 < 0 || nt < 1 || t0 + nt > ctx->Lt) fail(ED_ERR_ARG, "timeslice range [%d, %d) outside [0, %d)", t0, t0 + nt, ctx->Lt);
+    if (mode != ED_MODE_FP64 && mode != ED_MODE_MIXED) fail(ED_ERR_ARG, "unknown mode %d", mode);
+    // building the plan allocates its tables: it must already be built
+    require(!ctx->plan_dirty, ED_ERR_STATE,
+            "the plan is not built: call ed_compute or ed_query once after the last ed_set_* / ed_set_option call");
+    char* fixed = g_scratch_ptr[ctx->device];
+    require(fixed != nullptr, ED_ERR_STATE, "no scratch workspace on this device");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (mode == ED_MODE_FP64 || mixed_runs_fp64(ctx))
+      run_batches<double>(ctx, t0, nt, (double2*)out_device, nullptr, fixed, ED_NOALLOC_SCRATCH_BYTES);
+    else
+      run_batches<float>(ctx, t0, nt, (double2*)out_device, nullptr, fixed, ED_NOALLOC_SCRATCH_BYTES);
   });
 }
 

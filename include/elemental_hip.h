@@ -123,6 +123,30 @@ int ed_set_weights(ed_context* ctx, const double* coeff);
  * otherwise out is host memory and the call returns when it is filled. */
 int ed_compute(ed_context* ctx, int t0, int nt, int mode, void* out, int out_on_device);
 
+/* No-allocation form of ed_compute.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.
+ *   - out_device: a device array [nt][D][P][Ne][Ne] complex128; the call
+ *     enqueues on the context stream and returns without waiting.
+ *   - the displaced vectors go to a fixed workspace of
+ *     ED_NOALLOC_SCRATCH_BYTES compiled into the library (one per device,
+ *     held until the process ends) instead of the context's workspace;
+ *     timeslices run in batches that fit it.  A lattice whose single
+ *     timeslice needs more (slots x NeP x 3 x V3P x 8 bytes in the mixed
+ *     form, x 16 in FP64, NeP = Ne rounded up to 16, V3P = V3 rounded up to
+ *     32) fails with ED_ERR_UNSUPPORTED.
+ *   - the plan tables must be built: ed_compute or ed_query once after the
+ *     last ed_set_* / ed_set_option call; otherwise ED_ERR_STATE.
+ *   - nothing is allocated or freed and no event is recorded; the used part of
+ *     the workspace is cleared by a fill kernel on each call (the workspace
+ *     keeps the previous call's contents).
+ *   - every call on one device shares that device's workspace: calls that
+ *     overlap on one device (two streams, two contexts, two host threads)
+ *     race and give wrong results; order them on one stream.
+ * Results equal ed_compute's bit for bit. */
+#define ED_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int ed_compute_noalloc(ed_context* ctx, int t0, int nt, int mode, void* out_device);
+
 /* Block until all work enqueued on the context stream is done. */
 int ed_synchronize(ed_context* ctx);
 
