@@ -76,3 +76,54 @@ def test_device_eigenvectors_feed_elementals():
     assert np.abs(E - ref).max() / np.abs(ref).max() < 1e-12
     # V^dag V = 1 through the p = 0, d = 0 elemental
     assert np.abs(E[:, 0, 0] - np.eye(Ne)).max() < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["smooth", "haar"])
+def test_mixed_form_on_laplacian_modes_16c(kind):
+    """The mixed kernels (they run from V3 = 4096 on) fed with Laplacian low modes
+    from the device eigensolver at 16^3, Nev 48: smooth low modes give a large
+    coherent site-independent part of Q, the case the chunk-offset subtraction
+    of the FP16 phase contraction is for.  Eigenvalues against scipy eigsh,
+    residuals against the oracle Laplacian, then mixed elementals within 1e-6
+    of the oracle for every (d, p) at t = 0 and of the FP64 form at every t."""
+    from easydistillation_amd import _native
+
+    latt, Ne, nn = [16, 16, 16, 2], 48, 2
+    rng = np.random.default_rng(21)
+    U = S.gauge_np(rng, latt, kind)
+    gen = EigenvectorGenerator(latt, None, Ne, tol=1e-10)
+    gen.load_links(U)
+    V, vals = gen.calc_all()
+    _, rw = EO.lowest_modes(U[:, 0], Ne, tol=1e-11)
+    dl = np.abs(vals[0] - rw).max()
+    Vn = V.cpu().numpy()
+    res = max(
+        np.abs(np.moveaxis(EO.laplacian(np.moveaxis(Vn[t], 0, -1), U[:, t]), -1, 0)
+               - vals[t][:, None, None, None, None] * Vn[t]).max()
+        for t in range(latt[3])
+    )
+    say(f"16^3 {kind}: eigenvalues {vals[0][0]:.4f}..{vals[0][-1]:.4f}, |dlambda| {dl:.1e}, residual {res:.1e}")
+    assert dl < 1e-9 and res < 1e-8
+
+    moms = S.momenta_by_norm(2)
+    out = {}
+    for mode in ("fp64", "mixed"):
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode=mode)
+        g.load_arrays(torch.from_numpy(U).cuda(), V)
+        if mode == "mixed":
+            assert g.context.query(_native.ED_Q_MIXED_KERNEL) == 3
+            assert g.context.orthonormality() < 1e-6
+        out[mode] = g.calc_all()
+    ref = O.elemental_timeslice(Vn[0], U[:, 0], latt, nn, moms)
+
+    def blockwise(E, R):
+        d = np.abs(E - R).reshape(R.shape[0], R.shape[1], -1).max(-1)
+        r = np.abs(R).reshape(R.shape[0], R.shape[1], -1).max(-1)
+        row = r.max(1, keepdims=True)
+        return float((d / np.where(r >= 1e-8 * row, r, row)).max())
+
+    e64 = blockwise(out["fp64"][0], ref)
+    emx = blockwise(out["mixed"][0], ref)
+    emf = max(blockwise(out["mixed"][t], out["fp64"][t]) for t in range(latt[3]))
+    say(f"16^3 {kind}: vs oracle fp64 {e64:.2e} mixed {emx:.2e}; mixed vs fp64 {emf:.2e}")
+    assert e64 <= 1e-12 and emx <= 1e-6 and emf <= 1e-6
