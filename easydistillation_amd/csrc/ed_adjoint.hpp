@@ -6,7 +6,7 @@
 //   E[d,p,e,f] = w[e,f] sum_{s in splits(d)} c_s sum_{x,c} Phi_p(x) conj(W_{L_s}[e,x,c]) W_{R_s}[f,x,c].
 // With G = dL/dRe(E) + i dL/dIm(E) (so dL = Re sum conj(G) dE) and the same
 // convention for every input gradient, the reverse pass is
-//   Gamma_d[e,f,x] = sum_p w[e,f] G[d,p,e,f] conj(Phi_p(x))
+//   Gamma_d[e,f,x] = sum_p w[e,f] G[d,p,e,f] conj(Phi_p(x))   (opposite momenta paired)
 //   gW_{L_s}[e,x,c] += c_s sum_f conj(Gamma_d[e,f,x]) W_{R_s}[f,x,c]
 //   gW_{R_s}[f,x,c] += c_s sum_e Gamma_d[e,f,x]       W_{L_s}[e,x,c]
 // then, children before parents (W_child = nabla_mu W_parent), with
@@ -30,135 +30,248 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
 }
 
 // --------------------------------------------------------------------------
-// Weighted cotangent in the two orders the contraction adjoint reads:
-//   Acol[m][f][r] = w[f][r] G[m][f][r],   Arow[m][f][r] = w[r][f] G[m][r][f]
-// for every m = (tb, d, p) matrix of Ne x Ne.
-// grid (ceil(Ne/32), ceil(Ne/32), nmat), block (32, 8)
+// Weighted cotangent, momenta grouped, in the two orders the contraction
+// adjoint reads.  With A_p = w o G_p and Phi_p = c + i s, a group q is either
+// a pair (p1, p2) of opposite momenta, Phi_p2 = conj(Phi_p1), or one momentum
+// (p2 < 0), or a padding group (p1 < 0, zero); writing c, s for Re, Im Phi_p1(x),
+//   sum_{p in q} A_p conj(Phi_p) = S_q c + D_q s,
+//   S_q = A_p1 + A_p2,   D_q = -i (A_p1 - A_p2)     (A_p2 = 0 for one momentum),
+// so the phase sum of the adjoint costs two real-scaled terms per group.
+// Both orders hold [tb][d][q][S, D][f < Ne][r < NeP] (rows r contiguous,
+// zero for r >= Ne):
+//   Bcol[..][f][r] from A_p[f][r] = w[f][r] G_p[f][r],
+//   Brow[..][f][r] from A_p[r][f] = w[r][f] G_p[r][f].
+// grid (ceil(NeP/32), ceil(Ne/32), nz), z = blockIdx.z + zoff < nb D Q; block (32, 8)
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void adj_prep_kernel(const double2* __restrict__ G, const double* __restrict__ wgt,
-                                                       double2* __restrict__ Arow, double2* __restrict__ Acol, int Ne) {
-  __shared__ double2 tile[32][33];
-  const size_t off = (size_t)blockIdx.z * Ne * Ne;
+                                                       const int2* __restrict__ groups, double2* __restrict__ Brow,
+                                                       double2* __restrict__ Bcol, int Ne, int NeP, int P, int Q,
+                                                       int zoff) {
+  __shared__ double2 tS[32][33], tD[32][33];
+  const size_t ne2 = (size_t)Ne * Ne, bm = (size_t)Ne * NeP;
+  const int z = blockIdx.z + zoff, q = z % Q, td = z / Q;  // td = tb D + d
+  const int2 gq = groups[q];
+  // a padding group (gq.x < 0) has S = D = 0
+  const double2* G1 = G + ((size_t)td * P + max(gq.x, 0)) * ne2;
+  const double2* G2 = G + ((size_t)td * P + (gq.y >= 0 ? gq.y : max(gq.x, 0))) * ne2;
+  const double pair = gq.y >= 0 ? 1.0 : 0.0;
+  const double one = gq.x >= 0 ? 1.0 : 0.0;
+  double2* oS_c = Bcol + ((size_t)z * 2) * bm;
+  double2* oD_c = oS_c + bm;
+  double2* oS_r = Brow + ((size_t)z * 2) * bm;
+  double2* oD_r = oS_r + bm;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int r0 = blockIdx.x * 32, f0 = blockIdx.y * 32;
   for (int j = ty; j < 32; j += 8) {
-    const int f = f0 + j, r = r0 + tx;
-    if (f < Ne && r < Ne) {
-      double2 v = G[off + (size_t)f * Ne + r];
-      const double w = wgt ? wgt[(size_t)f * Ne + r] : 1.0;
-      Acol[off + (size_t)f * Ne + r] = make_double2(v.x * w, v.y * w);
+    // column order: element (f, r) = (f0 + j, r0 + tx)
+    int f = f0 + j, r = r0 + tx;
+    if (f < Ne && r < NeP) {
+      double2 sm = make_double2(0.0, 0.0), dm = make_double2(0.0, 0.0);
+      if (r < Ne) {
+        const size_t o = (size_t)f * Ne + r;
+        const double w = one * (wgt ? wgt[o] : 1.0);
+        const double2 a1 = G1[o], a2 = G2[o];
+        sm = make_double2(w * (a1.x + pair * a2.x), w * (a1.y + pair * a2.y));
+        dm = make_double2(w * (a1.y - pair * a2.y), -w * (a1.x - pair * a2.x));
+      }
+      oS_c[(size_t)f * NeP + r] = sm;
+      oD_c[(size_t)f * NeP + r] = dm;
     }
-    // tile of G rows r0.., columns f0..
-    const int rr = r0 + j, ff = f0 + tx;
-    double2 v = make_double2(0.0, 0.0);
-    if (rr < Ne && ff < Ne) {
-      v = G[off + (size_t)rr * Ne + ff];
-      const double w = wgt ? wgt[(size_t)rr * Ne + ff] : 1.0;
-      v = make_double2(v.x * w, v.y * w);
+    // row order: tile of G rows r0 + j, columns f0 + tx, written transposed
+    r = r0 + j;
+    f = f0 + tx;
+    double2 sm = make_double2(0.0, 0.0), dm = make_double2(0.0, 0.0);
+    if (r < Ne && f < Ne) {
+      const size_t o = (size_t)r * Ne + f;
+      const double w = one * (wgt ? wgt[o] : 1.0);
+      const double2 a1 = G1[o], a2 = G2[o];
+      sm = make_double2(w * (a1.x + pair * a2.x), w * (a1.y + pair * a2.y));
+      dm = make_double2(w * (a1.y - pair * a2.y), -w * (a1.x - pair * a2.x));
     }
-    tile[j][tx] = v;
+    tS[j][tx] = sm;
+    tD[j][tx] = dm;
   }
   __syncthreads();
   for (int j = ty; j < 32; j += 8) {
     const int f = f0 + j, r = r0 + tx;
-    if (f < Ne && r < Ne) Arow[off + (size_t)f * Ne + r] = tile[tx][j];
+    if (f < Ne && r < NeP) {
+      oS_r[(size_t)f * NeP + r] = tS[tx][j];
+      oD_r[(size_t)f * NeP + r] = tD[tx][j];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// out[m][j][i] = in[m][i][j] for nmat matrices of rows x cols complex128,
+// through 32 x 32 LDS tiles (the workspace [slot][site, colour][row] <->
+// [slot][row][site, colour] for the contraction adjoint).
+// grid (ceil(cols/32), ceil(rows/32), nmat), block (32, 8)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void adj_transpose_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                            int rows, int cols) {
+  __shared__ double2 tile[32][33];
+  const size_t off = (size_t)blockIdx.z * rows * cols;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < rows && c < cols) ? in[off + (size_t)r * cols + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j, r = r0 + tx;
+    if (c < cols && r < rows) out[off + (size_t)c * rows + r] = tile[tx][j];
   }
 }
 
 struct AdjArgs {
   Geom g;
-  int D, P;
-  const SplitTerm* splits;  // merged split terms of every derivative
-  const int2* dsplit;       // per derivative: (first, count)
-  const double2* phase;     // [V3][P] Phi_p(x)
+  int D, Q;
 };
 
 // --------------------------------------------------------------------------
-// Contraction adjoint.  A wave owns one site; lane r owns row r, so every
-// gW[n][x][c][r] it touches is written by this thread alone (no atomics):
-// as a left row it gathers sum_f conj(Gamma[r,f]) W_R[f], as a right row
-// sum_e Gamma[e,r] W_L[e].  NS bounds the split count of one derivative.
-// grid (ceil(V3 / 8), ceil(Ne / 64), TB), block 512
+// Contraction adjoint on the site-major workspace WT[slot][row][site][c]
+// (the transpose of W).  Lane = site x (64 consecutive sites per wave), and
+// the wave holds RB consecutive rows r0 .. r0+RB-1.  Per derivative d, first
+// for rows as left operands (Brow, gradient of the left slots), then as right
+// operands (Bcol, gradient of the right slots):
+//   Gamma[i](x) = sum_q S_q[f][r0+i] c_q(x) + D_q[f][r0+i] s_q(x)
+//   acc[i][k][c](x) += op(Gamma[i](x)) WT[src_k][f][x][c]        over f
+// (op = conj with the split's right slot as source, identity with its left),
+// then gWT[dst_k][r0+i][x][c] += coeff_k acc[i][k][c].  The cotangent values
+// S, D are the same for the whole wave: GC groups at a time are read with
+// scalar loads issued together (the group count is padded to a multiple of
+// GC with zero groups).  The phases c_q(x), s_q(x) and the W values are per
+// lane, coalesced over consecutive sites; each W load serves RB rows and is
+// issued one f ahead.  With QR > 0 the group count is exactly QR and every
+// group's phases stay in registers; QR = 0 takes any multiple of GC and
+// reloads the phases of groups past the first chunk.
+// Every gWT element a thread updates is its own (its site, the wave's rows):
+// no atomics.  phase [Q][V3P] (zero on padded sites and groups).  WT, gWT:
+// per timeslice [nW][NeP][V3P][3].  B: per timeslice [D][Q][2][Ne][NeP].
+// grid (ceil(V3P / 256), NeP / RB, TB), block 256
 // --------------------------------------------------------------------------
-template <int NS>
-__global__ __launch_bounds__(512) void adj_contract_kernel(const double2* __restrict__ W, double2* __restrict__ gW,
-                                                           const double2* __restrict__ Arow,
-                                                           const double2* __restrict__ Acol, AdjArgs a) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = blockIdx.x * 8 + wave;
-  const int r = blockIdx.y * 64 + lane;
+template <int NS, int RB, int GC, int QR>
+__global__ __launch_bounds__(256) void adj_contract_kernel(const double2* __restrict__ WT, double2* __restrict__ gWT,
+                                                           const double2* __restrict__ Brow,
+                                                           const double2* __restrict__ Bcol,
+                                                           const double2* __restrict__ phase,
+                                                           const SplitTerm* __restrict__ splits,
+                                                           const int2* __restrict__ dsplit, AdjArgs a) {
+  const int V3P = a.g.V3P, NeP = a.g.NeP, Ne = a.g.Ne, Q = a.Q;  // Q: a multiple of GC
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const bool live = x < V3P;
+  const int xc = live ? x : 0;
+  const int r0 = blockIdx.y * RB;  // < NeP
   const int tb = blockIdx.z;
-  if (s >= a.g.V3) return;  // whole wave: no barriers below
-  const int Ne = a.g.Ne;
-  const bool live = r < Ne;
-  const int rc = live ? r : 0;
-  const size_t slab = (size_t)a.g.nW * a.g.V3P * 3 * a.g.NeP;
-  const double2* Wt = W + (size_t)tb * slab;
-  double2* gWt = gW + (size_t)tb * slab;
-  const size_t ne2 = (size_t)Ne * Ne;
-  const double2* ph = a.phase + (size_t)s * a.P;
+  const size_t rowsz = (size_t)V3P * 3;  // one row of one slot
+  const size_t slotsz = (size_t)NeP * rowsz;
+  const double2* Wb = WT + (size_t)tb * a.g.nW * slotsz + (size_t)xc * 3;
+  double2* gWb = gWT + (size_t)tb * a.g.nW * slotsz + (size_t)xc * 3;
+  const size_t bm = (size_t)Ne * NeP;
+  constexpr int NPH = QR > 0 ? QR : GC;
+  double2 ph[NPH];  // phases of every group (QR > 0) or of the first chunk
+#pragma unroll
+  for (int q = 0; q < NPH; ++q) ph[q] = phase[(size_t)q * V3P + xc];
   for (int d = 0; d < a.D; ++d) {
-    const int2 fs = a.dsplit[d];
-    int nL[NS], nR[NS];
-    double cf[NS];
-    double2 accL[NS][3], accR[NS][3];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      if (k < fs.y) {
-        const SplitTerm st = a.splits[fs.x + k];
-        nL[k] = st.nL;
-        nR[k] = st.nR;
-        cf[k] = (double)st.coeff;
-      } else {
-        nL[k] = nR[k] = 0;
-        cf[k] = 0.0;
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) accL[k][c] = accR[k][c] = make_double2(0.0, 0.0);
-    }
-    const double2* Ar = Arow + ((size_t)tb * a.D + d) * a.P * ne2 + rc;
-    const double2* Ac = Acol + ((size_t)tb * a.D + d) * a.P * ne2 + rc;
-    for (int f = 0; f < Ne; ++f) {
-      double2 gr = make_double2(0.0, 0.0), gc = make_double2(0.0, 0.0);
-      for (int p = 0; p < a.P; ++p) {
-        const double2 phi = ph[p];
-        const size_t o = (size_t)p * ne2 + (size_t)f * Ne;
-        const double2 x1 = Ar[o], x2 = Ac[o];
-        const double2 t1 = cmulc(x1, phi), t2 = cmulc(x2, phi);
-        gr.x += t1.x;
-        gr.y += t1.y;
-        gc.x += t2.x;
-        gc.y += t2.y;
-      }
-      const double2 grc = make_double2(gr.x, -gr.y);
+    const int2 fs = dsplit[d];
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side) {
+      const double2* B = (side == 0 ? Brow : Bcol) + (size_t)(tb * a.D + d) * Q * 2 * bm + r0;
+      const double2* src[NS];
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        if (k < fs.y) {
+        const int n = k < fs.y ? (side == 0 ? splits[fs.x + k].nR : splits[fs.x + k].nL) : 0;
+        src[k] = Wb + (size_t)n * slotsz;
+      }
+      double2 acc[RB][NS][3];
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[i][k][c] = make_double2(0.0, 0.0);
+      double2 wn[NS][3];  // W of the next f
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) wn[k][c] = src[k][c];
+      for (int f = 0; f < Ne; ++f) {
+        double2 w[NS][3];
+        const int fn = min(f + 1, Ne - 1);
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const double2 wr = Wt[widx(a.g, nR[k], s, c, f)];
-            const double2 wl = Wt[widx(a.g, nL[k], s, c, f)];
-            const double2 u = cmul(grc, wr), v = cmul(gc, wl);
-            accL[k][c].x += cf[k] * u.x;
-            accL[k][c].y += cf[k] * u.y;
-            accR[k][c].x += cf[k] * v.x;
-            accR[k][c].y += cf[k] * v.y;
+            w[k][c] = wn[k][c];
+            wn[k][c] = src[k][(size_t)fn * rowsz + c];
+          }
+        double2 gam[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) gam[i] = make_double2(0.0, 0.0);
+        const double2* Bf = B + (size_t)f * NeP;
+        for (int q0 = 0; q0 < Q; q0 += GC) {
+          double2 sv[GC][RB], dv[GC][RB];
+#pragma unroll
+          for (int qq = 0; qq < GC; ++qq)
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+              sv[qq][i] = Bf[(size_t)(2 * (q0 + qq)) * bm + i];
+              dv[qq][i] = Bf[(size_t)(2 * (q0 + qq) + 1) * bm + i];
+            }
+#pragma unroll
+          for (int qq = 0; qq < GC; ++qq) {
+            // phase of group q0 + qq: from registers (selected without
+            // dynamic register indexing), or reloaded past them
+            double2 cs = ph[qq];
+#pragma unroll
+            for (int cc = 1; cc < NPH / GC; ++cc)
+              if (q0 == cc * GC) cs = ph[cc * GC + qq];
+            if (QR == 0 && q0 > 0) cs = phase[(size_t)(q0 + qq) * V3P + xc];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) {
+              gam[i].x = fma(sv[qq][i].x, cs.x, gam[i].x);
+              gam[i].y = fma(sv[qq][i].y, cs.x, gam[i].y);
+              gam[i].x = fma(dv[qq][i].x, cs.y, gam[i].x);
+              gam[i].y = fma(dv[qq][i].y, cs.y, gam[i].y);
+            }
+          }
+        }
+        if (side == 0) {
+#pragma unroll
+          for (int i = 0; i < RB; ++i) gam[i].y = -gam[i].y;
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          if (k < fs.y) {
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                acc[i][k][c].x = fma(gam[i].x, w[k][c].x, acc[i][k][c].x);
+                acc[i][k][c].x = fma(-gam[i].y, w[k][c].y, acc[i][k][c].x);
+                acc[i][k][c].y = fma(gam[i].x, w[k][c].y, acc[i][k][c].y);
+                acc[i][k][c].y = fma(gam[i].y, w[k][c].x, acc[i][k][c].y);
+              }
           }
         }
       }
-    }
-    if (live) {
+      if (live) {
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        if (k < fs.y) {
+        for (int k = 0; k < NS; ++k) {
+          if (k < fs.y) {
+            const SplitTerm st = splits[fs.x + k];
+            const double cf = (double)st.coeff;
+            double2* dst = gWb + (size_t)(side == 0 ? st.nL : st.nR) * slotsz;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            double2* pl = gWt + widx(a.g, nL[k], s, c, r);
-            double2 v = *pl;
-            *pl = make_double2(v.x + accL[k][c].x, v.y + accL[k][c].y);
-            double2* pr = gWt + widx(a.g, nR[k], s, c, r);
-            v = *pr;
-            *pr = make_double2(v.x + accR[k][c].x, v.y + accR[k][c].y);
+            for (int i = 0; i < RB; ++i) {
+              double2* pw = dst + (size_t)(r0 + i) * rowsz;
+#pragma unroll
+              for (int c = 0; c < 3; ++c) {
+                const double2 v = pw[c];
+                pw[c] = make_double2(v.x + cf * acc[i][k][c].x, v.y + cf * acc[i][k][c].y);
+              }
+            }
           }
         }
       }

@@ -184,9 +184,15 @@ struct ed_context {
   int nparent_all = 0;    // slots that have children (levels below num_nabla)
   DevBuf<SplitTerm> bsplits;
   DevBuf<int2> bdsplit;
-  DevBuf<double2> phase_all;  // [V3][P]
+  int nq = 0;                 // momentum groups: opposite pairs and single momenta
+  DevBuf<int2> bgroups;       // [nq] (p1, p2 or -1)
+  DevBuf<double2> phase_grp;  // [nq][V3P] Phi of each group's first momentum (0 on padded sites)
   DevBuf<int> child_of;       // [nparent_all][3]
-  DevBuf<double2> bW, bgW, bA, bG, bOutV, bOutU;
+  DevBuf<double2> bW, bA, bG, bOutV, bOutU;
+  // W transposed to [slot][row][site][colour] for the contraction adjoint; the
+  // same buffer then receives the gradient of W back in W's layout
+  DevBuf<double2> bWT;
+  DevBuf<double2> bgWT;  // gradient of W in the transposed layout
   bool bW_clear = false;
 
   // exact (complex128) eigenvectors for the FP64 form and the reverse pass
@@ -306,6 +312,17 @@ std::vector<SplitTerm> merged_splits(const std::vector<int>& t) {
     out.push_back(st);
   }
   return out;
+}
+
+// Shape of the contraction adjoint for a largest split count ns: rows per
+// wave (the accumulators are rows x splits x colours) and momentum groups
+// per chunk of scalar loads (rows x groups x 2 complex values in 64 SGPRs).
+#ifndef ED_ADJ_RB2
+#define ED_ADJ_RB2 4
+#endif
+void adj_shape(int ns, int& rb, int& gc) {
+  rb = ns <= 1 ? 4 : ns <= 2 ? ED_ADJ_RB2 : ns <= 4 ? 2 : 1;
+  gc = 8 / rb > 4 ? 4 : 8 / rb;
 }
 
 void build_plan(ed_context* c) {
@@ -494,8 +511,8 @@ void build_plan(ed_context* c) {
   HIPCHK(hipMemcpy(c->ph16.p, p16.data(), p16.size() * sizeof(_Float16), hipMemcpyHostToDevice));
 
   // reverse pass: merged splits of every derivative (no Hermitian pairing),
-  // Phi_p(x) of every listed momentum as [V3][P], and the child slot of every
-  // (parent slot, direction)
+  // the momentum groups and Phi of each group's first momentum as [nq][V3P],
+  // and the child slot of every (parent slot, direction)
   {
     std::vector<SplitTerm> bsp;
     std::vector<int2> bds(D);
@@ -507,18 +524,46 @@ void build_plan(ed_context* c) {
       bsp.insert(bsp.end(), terms.begin(), terms.end());
       c->bns_max = std::max(c->bns_max, (int)terms.size());
     }
-    std::vector<double2> pa((size_t)c->V3 * P);
+    // momentum groups of the adjoint's phase sum: each momentum pairs with an
+    // unused opposite one in the list (Phi_-n = conj(Phi_n)), else stands alone
+    std::vector<int2> grp;
+    {
+      std::vector<char> used(P, 0);
+      for (int p = 0; p < P; ++p) {
+        if (used[p]) continue;
+        used[p] = 1;
+        int partner = -1;
+        const int* n = &c->moms[3 * p];
+        if (n[0] || n[1] || n[2])
+          for (int o = p + 1; o < P && partner < 0; ++o)
+            if (!used[o] && c->moms[3 * o] == -n[0] && c->moms[3 * o + 1] == -n[1] && c->moms[3 * o + 2] == -n[2]) {
+              partner = o;
+              used[o] = 1;
+            }
+        grp.push_back(make_int2(p, partner));
+      }
+      // padded with zero groups to whole chunks of the adjoint's scalar loads,
+      // and up to 4 or 8 groups, whose phases the adjoint holds in registers
+      int rb, gc;
+      adj_shape(c->bns_max, rb, gc);
+      const size_t want = grp.size() <= 4 ? 4 : grp.size() <= 8 ? 8 : (grp.size() + gc - 1) / gc * gc;
+      while (grp.size() < want) grp.push_back(make_int2(-1, -1));
+    }
+    c->nq = (int)grp.size();
+    std::vector<double2> pa((size_t)c->V3P * c->nq, make_double2(0.0, 0.0));
     for (int z = 0; z < c->Lz; ++z)
       for (int y = 0; y < c->Ly; ++y)
         for (int x = 0; x < c->Lx; ++x) {
           const size_t st = ((size_t)z * c->Ly + y) * c->Lx + x;
-          for (int p = 0; p < P; ++p) {
+          for (int q = 0; q < c->nq; ++q) {
+            const int p = grp[q].x;
+            if (p < 0) continue;
             auto md = [](long a, long m) { long r = a % m; return r < 0 ? r + m : r; };
             const double fr = (double)md((long)c->moms[3 * p] * x, c->Lx) / c->Lx +
                               (double)md((long)c->moms[3 * p + 1] * y, c->Ly) / c->Ly +
                               (double)md((long)c->moms[3 * p + 2] * z, c->Lz) / c->Lz;
             const double ang = 2.0 * M_PI * fr;
-            pa[st * P + p] = make_double2(std::cos(ang), std::sin(ang));
+            pa[(size_t)q * c->V3P + st] = make_double2(std::cos(ang), std::sin(ang));
           }
         }
     c->nparent_all = c->disp ? 0 : level_start(c->num_nabla);
@@ -531,11 +576,13 @@ void build_plan(ed_context* c) {
       }
     c->bsplits.ensure(bsp.size());
     c->bdsplit.ensure(bds.size());
-    c->phase_all.ensure(pa.size());
+    c->bgroups.ensure(grp.size());
+    c->phase_grp.ensure(pa.size());
     c->child_of.ensure(child.size());
     HIPCHK(hipMemcpy(c->bsplits.p, bsp.data(), bsp.size() * sizeof(SplitTerm), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->bdsplit.p, bds.data(), bds.size() * sizeof(int2), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->phase_all.p, pa.data(), pa.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bgroups.p, grp.data(), grp.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->phase_grp.p, pa.data(), pa.size() * sizeof(double2), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->child_of.p, child.data(), child.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   c->plan_dirty = false;
@@ -903,8 +950,9 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
   const int D = c->D, Ne = c->Ne, P = (int)(c->moms.size() / 3);
   const size_t wslab = (size_t)D * c->V3P * 3 * c->NeP;
   const size_t gmat = (size_t)D * P * Ne * Ne;
+  const size_t bmat = (size_t)D * c->nq * 2 * Ne * c->NeP;  // grouped cotangent of one order, [D][nq][S, D][Ne][NeP]
   const size_t vt = (size_t)Ne * 3 * c->V3, ut = (size_t)27 * c->V3;
-  size_t per_t = (2 * wslab + 2 * gmat) * sizeof(double2);
+  size_t per_t = (3 * wslab + 2 * bmat) * sizeof(double2);
   if (!g_dev) per_t += gmat * sizeof(double2);
   if (!o_dev) per_t += ((gV ? vt : 0) + (gU ? ut : 0)) * sizeof(double2);
   int TB = c->tbatch_opt;
@@ -912,8 +960,8 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
     size_t budget = (size_t)std::max<int64_t>(1, c->ws_mb) << 20;
     // at most the reverse-pass buffers already held (reused) plus three
     // quarters of the free device memory, as for the forward batch
-    const size_t held =
-        (c->bW.n + c->bgW.n + c->bA.n + c->bG.n + c->bOutV.n + c->bOutU.n) * sizeof(double2);
+    const size_t held = (c->bW.n + c->bWT.n + c->bgWT.n + c->bA.n + c->bG.n + c->bOutV.n + c->bOutU.n) *
+                        sizeof(double2);
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = std::min(budget, held + free_b / 4 * 3);
     TB = (int)std::max<size_t>(1, budget / per_t);
@@ -926,13 +974,14 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
     HIPCHK(hipMemsetAsync(c->bW.p, 0, c->bW.n * sizeof(double2), c->stream));
     c->bW_clear = true;
   }
-  c->bgW.ensure(wslab * TB);
-  c->bA.ensure(2 * gmat * TB);
+  c->bWT.ensure(wslab * TB);
+  c->bgWT.ensure(wslab * TB);
+  c->bA.ensure(2 * bmat * TB);
   if (!g_dev) c->bG.ensure(gmat * TB);
   if (!o_dev && gV) c->bOutV.ensure(vt * TB);
   if (!o_dev && gU) c->bOutU.ensure(ut * TB);
-  double2* Arow = c->bA.p;
-  double2* Acol = c->bA.p + gmat * TB;
+  double2* Brow = c->bA.p;
+  double2* Bcol = c->bA.p + bmat * TB;
   const double2* U = c->U64.p;
 
   for (int tb0 = t0; tb0 < t0 + nt; tb0 += TB) {
@@ -949,47 +998,79 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
       HIPCHK(hipMemcpyAsync(c->bG.p, Gb, (size_t)nb * gmat * sizeof(double2), hipMemcpyHostToDevice, c->stream));
       Gb = c->bG.p;
     }
-    const size_t nmat = (size_t)nb * D * P, ne2 = (size_t)Ne * Ne;
-    for (size_t m0 = 0; m0 < nmat; m0 += 65535) {
-      const unsigned cnt = (unsigned)std::min<size_t>(65535, nmat - m0);
-      dim3 grid((Ne + 31) / 32, (Ne + 31) / 32, cnt);
-      ed::adj::adj_prep_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(Gb + m0 * ne2, c->use_wgt ? c->wgt.p : nullptr,
-                                                                     Arow + m0 * ne2, Acol + m0 * ne2, Ne);
+    const int nz = nb * D * c->nq;
+    for (int z0 = 0; z0 < nz; z0 += 65535) {
+      const unsigned cnt = (unsigned)std::min(65535, nz - z0);
+      dim3 grid((c->NeP + 31) / 32, (Ne + 31) / 32, cnt);
+      ed::adj::adj_prep_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(Gb, c->use_wgt ? c->wgt.p : nullptr, c->bgroups.p,
+                                                                     Brow, Bcol, Ne, c->NeP, P, c->nq, z0);
       HIPCHK(hipGetLastError());
     }
-    // contraction adjoint
-    HIPCHK(hipMemsetAsync(c->bgW.p, 0, (size_t)nb * wslab * sizeof(double2), c->stream));
+    // contraction adjoint on the transposed workspace [slot][row][site][colour]
+    const int wrows = 3 * c->V3P, wcols = c->NeP;
+    {
+      dim3 grid((wcols + 31) / 32, (wrows + 31) / 32, nb * D);
+      ed::adj::adj_transpose_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(c->bW.p, c->bWT.p, wrows, wcols);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemsetAsync(c->bgWT.p, 0, (size_t)nb * wslab * sizeof(double2), c->stream));
     ed::adj::AdjArgs a{};
     a.g = g;
     a.D = D;
-    a.P = P;
-    a.splits = c->bsplits.p;
-    a.dsplit = c->bdsplit.p;
-    a.phase = c->phase_all.p;
+    a.Q = c->nq;
     c->launch(ED_KERNEL_ADJOINT, [&] {
-      dim3 grid((c->V3 + 7) / 8, (Ne + 63) / 64, nb);
+      int rb, gc;
+      adj_shape(c->bns_max, rb, gc);
+      auto go = [&](auto ns, auto rbc) {
+        constexpr int NS = decltype(ns)::value, RB = decltype(rbc)::value;
+        constexpr int GC = 8 / RB > 4 ? 4 : 8 / RB;
+        dim3 grid((c->V3P + 255) / 256, c->NeP / RB, nb);
+        const double2* const Wp = c->bWT.p;
+        double2* const gp = c->bgWT.p;
+        if (c->nq == 4)
+          ed::adj::adj_contract_kernel<NS, RB, GC, 4><<<grid, 256, 0, c->stream>>>(
+              Wp, gp, Brow, Bcol, c->phase_grp.p, c->bsplits.p, c->bdsplit.p, a);
+        else if (c->nq == 8)
+          ed::adj::adj_contract_kernel<NS, RB, GC, 8><<<grid, 256, 0, c->stream>>>(
+              Wp, gp, Brow, Bcol, c->phase_grp.p, c->bsplits.p, c->bdsplit.p, a);
+        else
+          ed::adj::adj_contract_kernel<NS, RB, GC, 0><<<grid, 256, 0, c->stream>>>(
+              Wp, gp, Brow, Bcol, c->phase_grp.p, c->bsplits.p, c->bdsplit.p, a);
+      };
+      using I1 = std::integral_constant<int, 1>;
+      using I2 = std::integral_constant<int, 2>;
+      using I4 = std::integral_constant<int, 4>;
+      using I8 = std::integral_constant<int, 8>;
+      if (c->bns_max > 8) fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", c->bns_max);
       if (c->bns_max <= 1)
-        ed::adj::adj_contract_kernel<1><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+        go(I1{}, I4{});
+      else if (c->bns_max <= 2 && rb == 4)
+        go(I2{}, I4{});
+      else if (c->bns_max <= 2 && rb == 2)
+        go(I2{}, I2{});
       else if (c->bns_max <= 2)
-        ed::adj::adj_contract_kernel<2><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+        go(I2{}, I1{});
       else if (c->bns_max <= 4)
-        ed::adj::adj_contract_kernel<4><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
-      else if (c->bns_max <= 8)
-        ed::adj::adj_contract_kernel<8><<<grid, 512, 0, c->stream>>>(c->bW.p, c->bgW.p, Arow, Acol, a);
+        go(I4{}, I2{});
       else
-        fail(ED_ERR_UNSUPPORTED, "derivative with %d split terms", c->bns_max);
+        go(I8{}, I1{});
     });
+    {
+      dim3 grid((wrows + 31) / 32, (wcols + 31) / 32, nb * D);
+      ed::adj::adj_transpose_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(c->bgWT.p, c->bWT.p, wcols, wrows);
+      HIPCHK(hipGetLastError());
+    }
     // hop adjoint, deepest level first
     for (int k = c->num_nabla; k >= 1; --k) {
       const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
       dim3 grid((c->V3 + 3) / 4, nparent, nb);
-      ed::adj::adj_hop_kernel<<<grid, 256, 0, c->stream>>>(c->bgW.p, U, g, tb0, parent0, level_start(k));
+      ed::adj::adj_hop_kernel<<<grid, 256, 0, c->stream>>>(c->bWT.p, U, g, tb0, parent0, level_start(k));
       HIPCHK(hipGetLastError());
     }
     if (gU) {
       double2* o = o_dev ? gU : c->bOutU.p;
       dim3 grid((c->V3 + 3) / 4, 3, nb);
-      ed::adj::adj_link_kernel<<<grid, 256, 0, c->stream>>>(c->bW.p, c->bgW.p, c->child_of.p, c->nparent_all, o, g,
+      ed::adj::adj_link_kernel<<<grid, 256, 0, c->stream>>>(c->bW.p, c->bWT.p, c->child_of.p, c->nparent_all, o, g,
                                                              o_dev ? tl : 0, o_dev ? nt : nb);
       HIPCHK(hipGetLastError());
       if (!o_dev)
@@ -1000,7 +1081,7 @@ void run_backward(ed_context* c, int t0, int nt, const double2* G, bool g_dev, d
     if (gV) {
       double2* o = o_dev ? gV : c->bOutV.p;
       dim3 grid((3 * c->V3 + 31) / 32, (Ne + 31) / 32, nb);
-      ed::adj::adj_out_v_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(c->bgW.p, o, g, o_dev ? tl : 0);
+      ed::adj::adj_out_v_kernel<<<grid, dim3(32, 8), 0, c->stream>>>(c->bWT.p, o, g, o_dev ? tl : 0);
       HIPCHK(hipGetLastError());
       if (!o_dev)
         HIPCHK(hipMemcpyAsync(gV + (size_t)tl * vt, c->bOutV.p, (size_t)nb * vt * sizeof(double2),
