@@ -23,6 +23,8 @@ ED_OPT_EXACT_V = 6
 ED_OPT_MIXED_WS = 7
 ED_OPT_WIDE_WS = 8
 ED_OPT_ORTHO_GATE = 9
+ED_OPT_STRIP = 10
+ED_OPT_ITEM_ORDER = 11
 ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE, ED_KERNEL_ADJOINT = 0, 1, 2, 3
 
 # Every exported symbol and its ctypes signature (restype, argtypes).
@@ -46,6 +48,8 @@ SIGNATURES = {
     "ed_synchronize": (_I, [_VP]),
     "ed_orthonormality": (_I, [_VP, ctypes.POINTER(ctypes.c_double)]),
     "ed_set_option": (_I, [_VP, _I, ctypes.c_int64]),
+    "ed_get_option": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_int64)]),
+    "ed_tune": (_I, [_VP, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_double)]),
     "ed_profile_read": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "ed_profile_reset": (_I, [_VP]),
     "ed_query": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_int64)]),
@@ -65,7 +69,7 @@ SIGNATURES = {
     "ed_batch_time": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double)]),
     "ed_batch_tune": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double)]),
 }
-ED_BOPT_GRID_PER_CU, ED_BOPT_ORDER, ED_BOPT_GRAPH = 1, 2, 3
+ED_BOPT_GRID_PER_CU, ED_BOPT_ORDER, ED_BOPT_GRAPH, ED_BOPT_STRIP = 1, 2, 3, 4
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
 ED_Q_MIXED_F32_FLOPS_PER_T, ED_Q_MIXED_F16_FLOPS_PER_T = 7, 8
@@ -163,6 +167,17 @@ class Context:
 
     def set_option(self, option: int, value: int) -> None:
         self._chk("ed_set_option", self.lib.ed_set_option(self.h, option, int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        self._chk("ed_get_option", self.lib.ed_get_option(self.h, option, ctypes.byref(v)))
+        return v.value
+
+    def tune(self, mode: int, t0: int, nt: int, reps: int = 1) -> float:
+        """ed_tune: time the contraction's tile orders and kernels, keep the fastest; ms per run."""
+        ms = ctypes.c_double()
+        self._chk("ed_tune", self.lib.ed_tune(self.h, mode, t0, nt, reps, ctypes.byref(ms)))
+        return ms.value
 
     def compute(self, t0: int, nt: int, mode: int, out_ptr: int, out_on_device: bool) -> None:
         self._chk("ed_compute", self.lib.ed_compute(self.h, t0, nt, mode, _VP(out_ptr), int(out_on_device)))

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -171,7 +172,9 @@ struct ed_context {
   // k-steps), 2 warp-specialized kernel with FP16 stage 1 (up to 2 k-groups),
   // 0 uniform waves
   int mixed_ws = 1;
-  int wide_ws = 0;  // 64-bit workspace addresses in the warp-specialized kernel below 2^31 bytes too
+  int wide_ws = 0;
+  int strip = 0;       // e-tile rows per strip of the contraction's tile order (0: kernel default)
+  int item_order = 0;  // contraction items over the XCDs: 0 contiguous ranges, 1 round robin  // 64-bit workspace addresses in the warp-specialized kernel below 2^31 bytes too
   float vmax = 0.f;        // largest |re|, |im| of the stored eigenvectors
   double vv_dev = -1.0;    // FP64 max|V^dag V - 1| over timeslices of the stored eigenvectors
   int ortho_gate = 0;      // n > 0: ed_set_eigvecs refuses vv_dev > 10^-n
@@ -813,6 +816,8 @@ ContractArgs contract_args(const ed_context* c, int nb, int t_out0) {
   a.minus_idx = c->pidx.p + c->npb * 16;
   a.wgt = c->use_wgt ? c->wgt.p : nullptr;
   a.psum = c->psum.p;
+  a.strip = c->strip;
+  a.order = c->item_order;
   return a;
 }
 
@@ -1145,6 +1150,7 @@ struct ed_batch {
   std::vector<Item> items;
   int grid_per_cu = 2;  // persistent grid = grid_per_cu x CUs; 0: one workgroup per item
   int order = 0;        // item order of the grouped kernels
+  int strip = 0;        // tile-order strip height for every member (0: each context's own)
   int use_graph = 1;
   int ncu = 256;
   // kinds: 0 FP64 MFMA, 1 FP32 MFMA (mixed, FP32 stage 2), 2 mixed FP16 stage 2
@@ -1220,6 +1226,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
     void* W = kind == 0 ? (void*)ensure_workspace<double>(c, c->Lt) : (void*)ensure_workspace<float>(c, c->Lt);
     ed::GroupProblem gp{};
     gp.a = contract_args(c, c->Lt, 0);
+    if (b->strip > 0) gp.a.strip = b->strip;
     gp.W = W;
     if (kind == 0) {
       const size_t phn = (size_t)c->npb * 16 * c->V3P;
@@ -1269,6 +1276,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
   }
   fp.push_back((uint64_t)b->grid_per_cu);
   fp.push_back((uint64_t)b->order);
+  fp.push_back((uint64_t)b->strip);
   fp.push_back((uint64_t)b->wide);
   return fp;
 }
@@ -1685,6 +1693,14 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
         if (value < 0 || value > 15) fail(ED_ERR_ARG, "orthonormality gate must be in [0, 15]");
         ctx->ortho_gate = (int)value;
         break;
+      case ED_OPT_STRIP:
+        if (value < 0 || value > 64) fail(ED_ERR_ARG, "strip height must be in [0, 64]");
+        ctx->strip = (int)value;
+        break;
+      case ED_OPT_ITEM_ORDER:
+        if (value < 0 || value > 1) fail(ED_ERR_ARG, "item order must be 0 or 1");
+        ctx->item_order = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown option %d", option);
     }
@@ -1795,6 +1811,101 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
       case ED_Q_MIXED_KERNEL: *value = mixed_kernel_id(ctx); break;
       default: fail(ED_ERR_ARG, "unknown query %d", what);
     }
+  });
+}
+
+int ed_get_option(ed_context* ctx, int option, int64_t* value) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    if (!value) fail(ED_ERR_ARG, "value is null");
+    switch (option) {
+      case ED_OPT_HERMITIAN: *value = ctx->hermitian; break;
+      case ED_OPT_TBATCH: *value = ctx->tbatch_opt; break;
+      case ED_OPT_WS_MB: *value = ctx->ws_mb; break;
+      case ED_OPT_PROFILE: *value = ctx->profile; break;
+      case ED_OPT_MIXED_STAGE2: *value = ctx->mixed_stage2; break;
+      case ED_OPT_EXACT_V: *value = ctx->exact_v; break;
+      case ED_OPT_MIXED_WS: *value = ctx->mixed_ws; break;
+      case ED_OPT_WIDE_WS: *value = ctx->wide_ws; break;
+      case ED_OPT_ORTHO_GATE: *value = ctx->ortho_gate; break;
+      case ED_OPT_STRIP: *value = ctx->strip; break;
+      case ED_OPT_ITEM_ORDER: *value = ctx->item_order; break;
+      default: fail(ED_ERR_ARG, "unknown option %d", option);
+    }
+  });
+}
+
+int ed_tune(ed_context* ctx, int mode, int t0, int nt, int reps, double* best_ms) {
+  if (!ctx) return ED_ERR_ARG;
+  return guard(ctx, [&] {
+    require(ctx->lattice_set && ctx->gauge_set && ctx->eig_set && ctx->mom_set, ED_ERR_STATE,
+            "lattice, gauge, eigenvectors and momenta must be set before ed_tune");
+    if (t0 < 0 || nt < 1 || t0 + nt > ctx->Lt) fail(ED_ERR_ARG, "timeslice range [%d, %d) outside [0, %d)", t0, t0 + nt, ctx->Lt);
+    if (mode != ED_MODE_FP64 && mode != ED_MODE_MIXED) fail(ED_ERR_ARG, "unknown mode %d", mode);
+    if (reps < 1) fail(ED_ERR_ARG, "reps must be >= 1");
+    HIPCHK(hipSetDevice(ctx->device));
+    build_plan(ctx);
+    DevBuf<double2> out;
+    out.ensure((size_t)nt * ctx->D * (ctx->moms.size() / 3) * ctx->Ne * ctx->Ne);
+    hipEvent_t ev0, ev1;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    const int prof = ctx->profile;
+    ctx->profile = 0;
+    const bool fp = mode == ED_MODE_FP64 || mixed_runs_fp64(ctx);
+    auto run = [&] {
+      if (fp) run_batches<double>(ctx, t0, nt, out.p, nullptr);
+      else run_batches<float>(ctx, t0, nt, out.p, nullptr);
+    };
+    // candidate mixed kernels: each distinct kernel the routing gives for
+    // kinds 1 (warp-specialized), 0 (uniform waves) and 2 (FP16 stage 1)
+    std::vector<int> kinds;
+    std::vector<int> seen;
+    const int kind0 = ctx->mixed_ws;
+    for (int k : {1, 0, 2}) {
+      if (fp && k != kind0) continue;
+      ctx->mixed_ws = k;
+      const int id = fp ? -1 : mixed_kernel_id(ctx);
+      if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
+      seen.push_back(id);
+      kinds.push_back(k);
+    }
+    double best = 1e300;
+    int bk = kind0, bs = ctx->strip, bo = ctx->item_order;
+    std::exception_ptr err;
+    try {
+      for (int k : kinds)
+        for (int st : {1, 2, 4, 8})
+          for (int o = 0; o < 2; ++o) {
+            ctx->mixed_ws = k;
+            ctx->strip = st;
+            ctx->item_order = o;
+            run();  // warm-up (workspace of this kind)
+            HIPCHK(hipEventRecord(ev0, ctx->stream));
+            for (int r = 0; r < reps; ++r) run();
+            HIPCHK(hipEventRecord(ev1, ctx->stream));
+            HIPCHK(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            if (ms / reps < best) {
+              best = ms / reps;
+              bk = k;
+              bs = st;
+              bo = o;
+            }
+          }
+    } catch (...) {
+      err = std::current_exception();
+    }
+    ctx->mixed_ws = bk;
+    ctx->strip = bs;
+    ctx->item_order = bo;
+    ctx->profile = prof;
+    ++ctx->gen;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (err) std::rethrow_exception(err);
+    if (best_ms) *best_ms = best;
   });
 }
 
@@ -2001,6 +2112,10 @@ int ed_batch_set_option(ed_batch* b, int option, int64_t value) {
       case ED_BOPT_GRAPH:
         b->use_graph = value ? 1 : 0;
         break;
+      case ED_BOPT_STRIP:
+        if (value < 0 || value > 64) fail(ED_ERR_ARG, "strip height must be in [0, 64]");
+        b->strip = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown batch option %d", option);
     }
@@ -2016,6 +2131,7 @@ int ed_batch_get_option(ed_batch* b, int option, int64_t* value) {
       case ED_BOPT_GRID_PER_CU: *value = b->grid_per_cu; break;
       case ED_BOPT_ORDER: *value = b->order; break;
       case ED_BOPT_GRAPH: *value = b->use_graph; break;
+      case ED_BOPT_STRIP: *value = b->strip; break;
       default: fail(ED_ERR_ARG, "unknown batch option %d", option);
     }
   });
@@ -2047,22 +2163,27 @@ int ed_batch_tune(ed_batch* b, int reps, double* best_ms) {
   return bguard(b, [&] {
     if (reps < 1) fail(ED_ERR_ARG, "reps must be >= 1");
     const int grids[] = {1, 2, 4, 0};
+    const int strips[] = {1, 2, 4, 8};
     double best = 1e300;
-    int bg = b->grid_per_cu, bo = b->order;
+    int bg = b->grid_per_cu, bo = b->order, bs = b->strip;
     for (int gsel : grids)
-      for (int o = 0; o < 2; ++o) {
-        b->grid_per_cu = gsel;
-        b->order = o;
-        b->drop_graph();
-        const double ms = batch_time(b, reps);
-        if (ms < best) {
-          best = ms;
-          bg = gsel;
-          bo = o;
+      for (int o = 0; o < 2; ++o)
+        for (int st : strips) {
+          b->grid_per_cu = gsel;
+          b->order = o;
+          b->strip = st;
+          b->drop_graph();
+          const double ms = batch_time(b, reps);
+          if (ms < best) {
+            best = ms;
+            bg = gsel;
+            bo = o;
+            bs = st;
+          }
         }
-      }
     b->grid_per_cu = bg;
     b->order = bo;
+    b->strip = bs;
     b->drop_graph();
     if (best_ms) *best_ms = best;
   });

@@ -367,6 +367,8 @@ struct ContractArgs {
   const int* minus_idx;  // [npb*16] output index of -b or -1
   const double* wgt;     // [Ne][Ne] blending weights or nullptr
   const double* psum;    // [npb*16] sum_x Phi_b(x): V3 when b = 0 mod L, else 0
+  int strip;             // e-tile rows per strip of the tile order (0: the kernel's default)
+  int order;             // single-lattice item order: 0 contiguous ranges per XCD, 1 round robin
 };
 
 // Tile k of one (timeslice, derivative, momentum block): all nte x ntf tiles,
@@ -375,11 +377,11 @@ struct ContractArgs {
 // within an f-tile the strip's rows, so 32 consecutive items (one per CU of
 // an XCD, started together) read about TS e slabs and 32 / TS f slabs of the
 // workspace, which then hit in that XCD's L2 (TS = 4 for the mixed kernels,
-// 2 for the FP64/FP32 kernel, measured).
+// 2 for the FP64/FP32 kernel, measured, and the defaults; ContractArgs::strip
+// overrides them).
 constexpr int TSTRIP = 4;
 constexpr int TSTRIP_FP = 2;
-template <int TS = TSTRIP>
-__host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror, int& te, int& tf) {
+__host__ __device__ inline void decode_tile(int k, int nte, int ntf, bool mirror, int TS, int& te, int& tf) {
   int s0 = 0;
   if (mirror) {
     for (;;) {  // strip rows [s0, s0 + h), columns tf >= s0, te <= tf
@@ -425,7 +427,7 @@ __device__ __forceinline__ void decode_item(const ContractArgs& a, int w, int& t
   const bool mirror = a.dplan[di].mirror;
   const int per = mirror ? a.nte * (a.nte + 1) / 2 : a.nte * a.ntf;
   pb = r / per;
-  decode_tile<TS>(r - pb * per, a.nte, a.ntf, mirror, te, tf);
+  decode_tile(r - pb * per, a.nte, a.ntf, mirror, a.strip > 0 ? a.strip : TS, te, tf);
 }
 
 constexpr int SC = 16;      // sites per chunk (4 k-steps of the stage-2 MFMA)
@@ -439,6 +441,13 @@ constexpr int NBW = 16 / NWAVE;   // stage-2 N-blocks (eigenvector rows e) per w
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   const int per = nblocks >> 3;  // nblocks is a multiple of 8
   return (bid & 7) * per + (bid >> 3);
+}
+
+// Work item of workgroup bid in a single-lattice launch: contiguous ranges
+// of items per XCD (the hardware hands workgroup bid to XCD bid mod 8), or
+// the items round robin over the XCDs.
+__device__ __forceinline__ int item_index(const ContractArgs& a, int bid, int nblocks) {
+  return a.order ? bid : xcd_remap(bid, nblocks);
 }
 
 // One workgroup (8 waves, two per SIMD) owns a 16 x 16 (e, f) tile, one
@@ -1494,7 +1503,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_kernel(const c_t<T>* _
                                                                  const T* __restrict__ phi, double2* __restrict__ out,
                                                                  ContractArgs a) {
   __shared__ c_t<T> Qs[2][SC][16][16];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = item_index(a, blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
   contract_body<T, NKS>(W, phr, phi, out, a, w, Qs);
 }
@@ -1506,7 +1515,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_f16_kernel(const
                                                                             ContractArgs a) {
   __shared__ float2 Qs[2][SC2][16][16];
   __shared__ float wmax[2][NWAVE];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = item_index(a, blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
   contract_mixed_body<NKS>(W, ph16, out, a, w, Qs, wmax);
 }
@@ -1519,7 +1528,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed_ws_kernel(const 
   __shared__ float2 Qs[2][SC2][QSITE];
   __shared__ float wmax[2][NWAVE];
   __shared__ half8 PHs[2][4][64];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = item_index(a, blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
   contract_mixed_ws_body<NKS, WIDE>(W, ph16, out, a, w, Qs, wmax, PHs);
 }

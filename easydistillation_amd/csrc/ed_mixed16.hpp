@@ -376,7 +376,7 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed16_kernel(const u
                                                                           ContractArgs a) {
   __shared__ __align__(16) float Qf[QFLOATS];
   __shared__ float wmax[2][NWAVE];
-  const int w = xcd_remap(blockIdx.x, gridDim.x);
+  const int w = item_index(a, blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
   contract_mixed16_body<NKS>(W, ph16, out, a, w, Qf, wmax);
 }

@@ -185,7 +185,13 @@ int ed_synchronize(ed_context* ctx);
  *   ED_OPT_ORTHO_GATE     n in [1, 15]: ed_set_eigvecs fails with ED_ERR_ARG (and the context
  *                         holds no eigenvectors) when the FP64 max|V^dag V - 1| of the stored
  *                         eigenvectors exceeds 10^-n; 0 (default): measured and reported by
- *                         ed_orthonormality only. */
+ *                         ed_orthonormality only.
+ *   ED_OPT_STRIP          tile order of the contraction: its 16 x 16 (e, f) tiles run in strips
+ *                         of this many e-tile rows, f-tile by f-tile inside a strip (0, default:
+ *                         4 for the mixed kernels, 2 for the FP64 / FP32 kernel).  Chosen by
+ *                         ed_tune.
+ *   ED_OPT_ITEM_ORDER     contraction work items over the 8 XCDs: 0 (default) contiguous ranges
+ *                         per XCD, 1 round robin.  Chosen by ed_tune. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
@@ -195,9 +201,22 @@ enum ed_option {
   ED_OPT_EXACT_V = 6,
   ED_OPT_MIXED_WS = 7,
   ED_OPT_WIDE_WS = 8,
-  ED_OPT_ORTHO_GATE = 9
+  ED_OPT_ORTHO_GATE = 9,
+  ED_OPT_STRIP = 10,
+  ED_OPT_ITEM_ORDER = 11
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
+int ed_get_option(ed_context* ctx, int option, int64_t* value);
+
+/* Autotuner of the contraction (no counterpart in the reference; the caller
+ * is easydistillation_amd.tune, which keeps the choices per shape in a JSON
+ * file): times ed_compute of timeslices t0 .. t0+nt-1 into a device buffer,
+ * reps runs after one warm-up, for every candidate -- tile-order strip
+ * height (ED_OPT_STRIP 1, 2, 4, 8) x item order over the XCDs
+ * (ED_OPT_ITEM_ORDER 0, 1) x, in the mixed form, each distinct mixed kernel
+ * that ED_OPT_MIXED_WS 1, 0 and 2 select for this shape -- leaves the fastest
+ * set on the context and returns its time per run in *best_ms. */
+int ed_tune(ed_context* ctx, int mode, int t0, int nt, int reps, double* best_ms);
 
 /* Kernel timing collected while ED_OPT_PROFILE = 1, per kernel family:
  *   ED_KERNEL_STENCIL  covariant hops (one launch per derivative level),
@@ -311,14 +330,15 @@ int ed_get_gauge(ed_context* ctx, void* U, int on_device);
  *   ed_batch_time: mean duration (HIP events) of reps runs after one warm-up.
  *   ed_batch_tune: autotuner -- times every launch shape (persistent grid of
  *                 1, 2 or 4 workgroups per CU, or one workgroup per work item)
- *                 and item order (contiguous ranges per XCD, or round robin
- *                 over XCDs) with reps runs each, keeps the fastest and
- *                 returns its time. */
+ *                 x item order (contiguous ranges per XCD, or round robin
+ *                 over XCDs) x tile-order strip height (1, 2, 4, 8) with reps
+ *                 runs each, keeps the fastest and returns its time. */
 typedef struct ed_batch ed_batch;
 enum ed_batch_option {
   ED_BOPT_GRID_PER_CU = 1, /* persistent workgroups per CU (0: one per work item; default 2) */
   ED_BOPT_ORDER = 2,       /* 0: contiguous item ranges per XCD (default), 1: round robin     */
-  ED_BOPT_GRAPH = 3        /* 1: capture and replay a hipGraph (default), 0: direct launches  */
+  ED_BOPT_GRAPH = 3,       /* 1: capture and replay a hipGraph (default), 0: direct launches  */
+  ED_BOPT_STRIP = 4        /* tile-order strip height for every member (0: each context's own) */
 };
 int ed_batch_create(int device, ed_batch** b);
 int ed_batch_destroy(ed_batch* b);

@@ -143,8 +143,7 @@ TILE_ORDER_SRC = r"""
 #include <utility>
 // every tile of an nt x nt grid (or of its upper triangle) exactly once, in
 // strips of TS e-tile rows walked f-tile by f-tile
-template <int TS>
-int check() {
+int check(int TS) {
   for (int nt = 1; nt <= 40; ++nt)
     for (int m = 0; m < 2; ++m) {
       const int per = m ? nt * (nt + 1) / 2 : nt * nt;
@@ -152,7 +151,7 @@ int check() {
       int prev_strip = 0;
       for (int k = 0; k < per; ++k) {
         int te = -1, tf = -1;
-        ed::decode_tile<TS>(k, nt, nt, m, te, tf);
+        ed::decode_tile(k, nt, nt, m, TS, te, tf);
         if (te < 0 || te >= nt || tf < 0 || tf >= nt || (m && te > tf)) return 1;
         if (te / TS < prev_strip) return 2;  // strips in order
         prev_strip = te / TS;
@@ -163,13 +162,14 @@ int check() {
   return 0;
 }
 int main() {
-  if (int r = check<ed::TSTRIP>()) return r;
-  if (int r = check<ed::TSTRIP_FP>()) return 10 + r;
+  const int strips[] = {1, 2, 3, 4, 8, ed::TSTRIP, ed::TSTRIP_FP};
+  for (int i = 0; i < 7; ++i)
+    if (int r = check(strips[i])) return 10 * i + r;
   int te, tf;
-  ed::decode_tile<4>(5, 13, 13, false, te, tf);  // strip 0: (0,0) (1,0) (2,0) (3,0) (0,1) (1,1)
-  if (te != 1 || tf != 1) return 4;
-  ed::decode_tile<4>(3, 13, 13, true, te, tf);   // triangle strip 0: (0,0) (0,1) (1,1) (0,2)
-  if (te != 0 || tf != 2) return 5;
+  ed::decode_tile(5, 13, 13, false, 4, te, tf);  // strip 0: (0,0) (1,0) (2,0) (3,0) (0,1) (1,1)
+  if (te != 1 || tf != 1) return 100;
+  ed::decode_tile(3, 13, 13, true, 4, te, tf);   // triangle strip 0: (0,0) (0,1) (1,1) (0,2)
+  if (te != 0 || tf != 2) return 101;
   std::printf("ok\n");
   return 0;
 }
@@ -225,3 +225,37 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_tune_table_round_trip_and_keys(tmp_path):
+    """The tuner's JSON table (no GPU): atomic save/load, and shape keys that
+    separate every plan-relevant difference."""
+    from types import SimpleNamespace
+
+    from easydistillation_amd.tune import load_table, save_table, shape_key
+
+    path = str(tmp_path / "sub" / "tune.json")
+    assert load_table(path) == {}
+    save_table(path, {"a": {"strip": 2}})
+    assert load_table(path) == {"a": {"strip": 2}}
+    (tmp_path / "bad.json").write_text("not json")
+    assert load_table(str(tmp_path / "bad.json")) == {}
+
+    def gen(**kw):
+        base = dict(latt_size=[16, 16, 16, 8], Ne=40, num_nabla=2, num_momentum=2,
+                    momentum_list=[(0, 0, 0), (1, 0, 0)], stocastic_coeff=None)
+        base.update(kw)
+        return SimpleNamespace(**base)
+
+    keys = {
+        shape_key(gen(), "mixed"),
+        shape_key(gen(), "fp64"),
+        shape_key(gen(Ne=41), "mixed"),
+        shape_key(gen(num_nabla=1), "mixed"),
+        shape_key(gen(latt_size=[16, 16, 8, 8]), "mixed"),
+        shape_key(gen(momentum_list=[(0, 0, 0), (0, 1, 0)]), "mixed"),
+        shape_key(gen(stocastic_coeff=np.ones((40, 40))), "mixed"),
+    }
+    assert len(keys) == 7
+    # Lt does not change the per-timeslice contraction
+    assert shape_key(gen(latt_size=[16, 16, 16, 64]), "mixed") == shape_key(gen(), "mixed")
