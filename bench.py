@@ -33,13 +33,20 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per contraction launch read from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of
 # "--workload C --steps 1 --warmup 0 --calibrate-copy 4096", scaled by the factors the 4 GiB copy
 # in the same run gives (2.00 x FETCH_SIZE, 1.00 x WRITE_SIZE); the summary and the commands are in
-# profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, warp-specialized).
+# profiles/elemental_32c64_nev200_hbm_traffic.md.  Keyed by (workload, form, mixed kernel id
+# (ED_Q_MIXED_KERNEL; None for FP64), strip height, item order) -- the tile order changes the bytes.
 TRAFFIC_MEASURED = {
-    ("C", "mixed", 1): (127743801.0 * 2.0 + 3239083.4) * 1024,  # contract_mixed_ws_kernel<3>, 8 timeslices
-    ("C", "fp64", 1): (434679524.2 * 2.0 + 2751779.8) * 1024,  # contract_kernel<double, 3>, 8 timeslices
-    ("C", "fp64", 0): (434679524.2 * 2.0 + 2751779.8) * 1024,
+    ("C", "mixed", 3, 4, 0): (127743801.0 * 2.0 + 3239083.4) * 1024,  # contract_mixed_ws_kernel<3>, 8 timeslices
+    ("C", "fp64", None, 2, 0): (434679524.2 * 2.0 + 2751779.8) * 1024,  # contract_kernel<double, 3>, 8 timeslices
+    # contract_mixed16_kernel<2> (the tuned mixed kernel), 8 timeslices:
+    # profiles/elemental_32c64_nev200_tuned_kernel_trace.md
+    ("C", "mixed", 5, 4, 1): (178699000.0 * 2.0 + 3328990.0) * 1024,
 }
-TRAFFIC_SOURCE = "profiles/elemental_32c64_nev200_hbm_traffic.md"
+TRAFFIC_SOURCE = {
+    ("C", "mixed", 3, 4, 0): "profiles/elemental_32c64_nev200_hbm_traffic.md",
+    ("C", "fp64", None, 2, 0): "profiles/elemental_32c64_nev200_hbm_traffic.md",
+    ("C", "mixed", 5, 4, 1): "profiles/elemental_32c64_nev200_tuned_kernel_trace.md",
+}
 # the library's ED_Q_MIXED_KERNEL answer -> the kernel the mixed form runs
 MIXED_KERNEL = {
     0: "contract_kernel<double, 3>",
@@ -445,6 +452,9 @@ def main():
                     help="mixed form's FP16 stage-2 kernel: 1 warp-specialized with FP32 stage 1 (default), "
                     "2 warp-specialized with FP16 stage 1, 0 uniform waves")
     ap.add_argument("--tbatch", type=int, default=0, help="timeslices per contraction launch (0: the library's choice)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (easydistillation_amd.tune) and run the library defaults / --mixed-ws")
+    ap.add_argument("--tune-cache", default=None, help="JSON table of tuned shapes (default: the tuner's own)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--ref-loop-t", type=int, default=16,
@@ -515,6 +525,13 @@ def main():
         if args.tbatch > 0:
             g.set_option(_native.ED_OPT_TBATCH, args.tbatch)
         g.load_arrays(U, V)
+        tuned = None
+        if not args.no_tune:
+            from easydistillation_amd.tune import autotune
+
+            t1 = time.perf_counter()
+            tuned = autotune(g, cache=args.tune_cache)
+            log(f"{form}: tuned contraction {tuned} in {time.perf_counter() - t1:.1f} s")
         out = torch.empty((Lt, D, P, Ne, Ne), dtype=torch.complex128, device=dev)
         for i in range(args.warmup):
             t1 = time.perf_counter()
@@ -562,6 +579,9 @@ def main():
             "stencil_ms_per_step": h_ms / args.steps,
             "stage_ms_per_step": s_ms / args.steps,
             "mixed_kernel": mixed_kernel,
+            "strip": g.context.get_option(_native.ED_OPT_STRIP),
+            "item_order": g.context.get_option(_native.ED_OPT_ITEM_ORDER),
+            "tuned": tuned,
         }
         log(f"{form}: {json.dumps(results[form])}")
         outs[form] = out
@@ -613,7 +633,9 @@ def main():
         log(f"reference-style loop: {json.dumps(ref_loop)}")
         del g, data
     # the measured bytes hold for the library's own timeslice batch (8 per launch at this workload)
-    traffic_key = (args.workload, head, args.mixed_ws) if args.tbatch == 0 else None
+    strip_eff = r["strip"] or (4 if head == "mixed" and r["mixed_kernel"] else 2)
+    all_f16 = head == "mixed" and r["mixed_kernel"] == 5
+    traffic_key = (args.workload, head, r["mixed_kernel"], strip_eff, r["item_order"]) if args.tbatch == 0 else None
     line = {
         "metric": METRIC,
         "value": r["elementals_per_s"],
@@ -649,15 +671,19 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "achieved": r["contract_tflops_executed"],
-            "peak": PEAK[head],
+            # the FP16 stage-1 kernel executes only FP16 MFMAs: quoted against the FP16 peak
+            "achieved": r["contract_fp16_tflops"] if all_f16 else r["contract_tflops_executed"],
+            "peak": PEAK_FP16 if all_f16 else PEAK[head],
             "unit": "TFLOP/s",
-            "frac": r["contract_tflops_executed"] / PEAK[head],
+            "frac": (r["contract_fp16_tflops"] / PEAK_FP16) if all_f16 else r["contract_tflops_executed"] / PEAK[head],
             "traffic": TRAFFIC_MEASURED.get(traffic_key),
-            "traffic_source": TRAFFIC_SOURCE if traffic_key in TRAFFIC_MEASURED else None,
+            "traffic_source": TRAFFIC_SOURCE.get(traffic_key),
             "kernel": MIXED_KERNEL[r["mixed_kernel"]] if head == "mixed" else "contract_kernel<double, 3>",
             "flops_basis": (
-                "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
+                "executed FP16 MFMA flops of the factorized contraction per launch (split sum and phase "
+                "contraction both on FP16 MFMA with hi/lo operand splits)"
+                if all_f16
+                else "executed MFMA flops of the factorized contraction per launch; the mixed kernel's FP16 "
                 "MFMA flops are converted to FP32-equivalent at the FP32:FP16 peak ratio 157.3:2500"
                 if head == "mixed"
                 else "executed FP64 MFMA flops of the factorized contraction per launch"

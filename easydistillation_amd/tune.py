@@ -10,6 +10,9 @@ timing again.  The reference has no tuner: its ``calc(t)``
 (lattice/generator/elemental.py:290-338) runs one fixed einsum path.
 
 The table lives at ``$ED_TUNE_CACHE`` or ``~/.cache/easydistillation_amd/tune.json``.
+Shapes missing there are looked up in ``tune_gfx950.json`` beside this module,
+the choices this tuner measured for the benchmark shapes on MI355X; a shape in
+neither is timed.
 """
 
 from __future__ import annotations
@@ -26,6 +29,9 @@ TUNED_OPTIONS = {
     "item_order": _native.ED_OPT_ITEM_ORDER,
     "mixed_ws": _native.ED_OPT_MIXED_WS,
 }
+
+
+SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.json")
 
 
 def default_cache_path() -> str:
@@ -66,20 +72,22 @@ def autotune(gen, mode: Optional[str] = None, cache: Optional[str] = None, nt: O
     """Pick the contraction shape for ``gen`` (a loaded ElementalGenerator) and apply it.
 
     Returns the chosen options with ``"source"``: ``"table"`` (read from the
-    JSON table) or ``"timed"`` (measured now and stored).  ``nt`` timeslices
-    (default: up to 8) are timed from t = 0.
+    JSON table), ``"shipped"`` (from the package's table) or ``"timed"``
+    (measured now and stored).  ``nt`` timeslices (default: up to 8) are
+    timed from t = 0.
     """
     if not gen._loaded:
         raise RuntimeError("load the generator (load or load_arrays) before tuning")
     mode = mode or gen.mode
     path = cache or default_cache_path()
     key = shape_key(gen, mode)
-    table = load_table(path)
-    entry = table.get(key)
-    if entry is not None and not retune:
-        for name, opt in TUNED_OPTIONS.items():
-            gen.set_option(opt, int(entry[name]))
-        return dict(entry, source="table")
+    if not retune:
+        for source, table in (("table", load_table(path)), ("shipped", load_table(SHIPPED))):
+            entry = table.get(key)
+            if entry is not None:
+                for name, opt in TUNED_OPTIONS.items():
+                    gen.set_option(opt, int(entry[name]))
+                return dict(entry, source=source)
     Lt = gen.latt_size[3]
     n = min(Lt, nt or 8)
     ms = gen.context.tune(gen._mode_code(mode), 0, n, reps)
