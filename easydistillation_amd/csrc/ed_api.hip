@@ -162,6 +162,14 @@ struct ed_context {
   float cmax = 1.f;                   // largest |split coefficient|
   DevBuf<DPlan> dplan;
   DevBuf<SplitTerm> splits;
+  // unit plan of the FP16 stage-1 kernel (build_plan): every coefficient +1
+  int h16_unit_opt = 1;               // ED_OPT_H16_UNIT
+  bool unit_ok = false;
+  bool ws_rot = false;                // the unit plan needs slot n stored times i^|n|
+  int nsplit_u_max = 1;               // largest split count of the unit plan
+  int64_t h16u_groups_sum[2] = {0, 0};
+  DevBuf<DPlan> dplan_u;
+  DevBuf<SplitTerm> splits_u;
   DevBuf<int> pidx;  // plus [npb*16] then minus [npb*16]
   DevBuf<double> psum;  // [npb*16] sum over sites of Phi_b
   DevBuf<double> ph64;
@@ -172,9 +180,9 @@ struct ed_context {
   // k-steps), 2 warp-specialized kernel with FP16 stage 1 (up to 2 k-groups),
   // 0 uniform waves
   int mixed_ws = 1;
-  int wide_ws = 0;
+  int wide_ws = 0;     // 64-bit workspace addresses in the warp-specialized kernel below 2^31 bytes too
   int strip = 0;       // e-tile rows per strip of the contraction's tile order (0: kernel default)
-  int item_order = 0;  // contraction items over the XCDs: 0 contiguous ranges, 1 round robin  // 64-bit workspace addresses in the warp-specialized kernel below 2^31 bytes too
+  int item_order = 0;  // contraction items over the XCDs: 0 contiguous ranges, 1 round robin
   float vmax = 0.f;        // largest |re|, |im| of the stored eigenvectors
   double vv_dev = -1.0;    // FP64 max|V^dag V - 1| over timeslices of the stored eigenvectors
   int ortho_gate = 0;      // n > 0: ed_set_eigvecs refuses vv_dev > 10^-n
@@ -236,6 +244,7 @@ struct ed_context {
     g.NeP = NeP;
     g.nW = nW;
     g.wsh = 0;
+    g.rot = 0;
     return g;
   }
 
@@ -496,6 +505,46 @@ void build_plan(ed_context* c) {
     }
     c->items_t = items;
   }
+
+  // Unit plan of the FP16 stage-1 kernel.  A merged split's coefficient is
+  // (-1)^{|nR|} times a positive multiplicity (merged_splits).  With slot n
+  // of the workspace stored times i^{|n|} (the hop multiplies each child by
+  // i), conj(L'') R'' = i^{|nR| - |nL|} conj(L) R = (-1)^{|nR|} i^{-|d|}
+  // conj(L) R since |nL| + |nR| = |d|: every split then enters as many
+  // times as its multiplicity with coefficient +1, and the result is
+  // multiplied by i^{|d|}.  Displacement terms have coefficient 1 and need no
+  // rotation.
+  {
+    std::vector<DPlan> dplu = dpl;
+    std::vector<SplitTerm> spu;
+    bool ok = true;
+    c->nsplit_u_max = 1;
+    c->h16u_groups_sum[0] = c->h16u_groups_sum[1] = 0;
+    for (DPlan& p : dplu) {
+      const int first = (int)spu.size();
+      for (int k = p.first; k < p.first + p.nsplit; ++k) {
+        const SplitTerm& st = sp[k];
+        const int sgn = c->disp ? 1 : (((int)derivative_tuple(st.nR).size() & 1) ? -1 : 1);
+        const float mult = st.coeff * (float)sgn;
+        const int m = (int)mult;
+        if ((float)m != mult || m < 1 || m > 4) ok = false;
+        for (int r = 0; ok && r < m; ++r) spu.push_back(SplitTerm{st.nL, st.nR, 1.f, 0});
+      }
+      p.first = first;
+      p.nsplit = (int)spu.size() - first;
+      p.rot = c->disp ? 0 : ((int)derivative_tuple(p.d).size() & 3);
+      c->nsplit_u_max = std::max(c->nsplit_u_max, p.nsplit);
+      c->h16u_groups_sum[p.mirror] += (3 * p.nsplit + 7) / 8;
+    }
+    c->unit_ok = ok && !dplu.empty();
+    c->ws_rot = c->unit_ok && !c->disp;
+    if (c->unit_ok) {
+      c->dplan_u.ensure(dplu.size());
+      c->splits_u.ensure(spu.size());
+      HIPCHK(hipMemcpy(c->dplan_u.p, dplu.data(), dplu.size() * sizeof(DPlan), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->splits_u.p, spu.data(), spu.size() * sizeof(SplitTerm), hipMemcpyHostToDevice));
+    }
+  }
   c->dplan.ensure(dpl.size());
   c->splits.ensure(std::max<size_t>(1, sp.size()));
   c->pidx.ensure(idx.size());
@@ -696,7 +745,10 @@ bool mixed_runs_fp64(const ed_context* c) { return c->mixed_stage2 < 0 && c->V3 
 // split-colour pairs (five split terms) per derivative, and V3 >= 4096 (on
 // smaller rough lattices the 22-bit workspace pairs push blocks that are
 // small against their terms past 1e-6; there the FP32 stage 1 runs)
-int h16_groups(const ed_context* c) { return (3 * c->nsplit_max + 7) / 8; }
+int h16_unit_groups(const ed_context* c) { return (3 * c->nsplit_u_max + 7) / 8; }
+// the unit plan runs when its k-groups fit the kernel
+bool h16_unit(const ed_context* c) { return c->h16_unit_opt && c->unit_ok && h16_unit_groups(c) <= 2; }
+int h16_groups(const ed_context* c) { return h16_unit(c) ? h16_unit_groups(c) : (3 * c->nsplit_max + 7) / 8; }
 bool use_h16(const ed_context* c) {
   return use_f16_stage2(c) && ws_fits32(c) && c->mixed_ws == 2 && h16_groups(c) <= 2 && c->V3 >= 4096;
 }
@@ -765,7 +817,10 @@ int auto_tbatch(const ed_context* c, int nt) {
 template <typename T, class L>
 void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t st, L&& launch) {
   Geom g = c->geom(c->slots());
-  if (std::is_same<T, ed::hsplit>::value) g.wsh = c->wsh;
+  if (std::is_same<T, ed::hsplit>::value) {
+    g.wsh = c->wsh;
+    g.rot = h16_unit(c) && c->ws_rot ? 1 : 0;
+  }
   launch(ED_KERNEL_STAGE, [&] {
     dim3 grid((3 * c->V3 + 63) / 64, (c->Ne + 63) / 64, nb);
     if (sizeof(T) == 8 && c->exact_v && c->v64_set)
@@ -885,6 +940,11 @@ void run_batches(ed_context* c, int t0, int nt, double2* out_dev, void* out_host
       c->wsh = h16_shift(c);
       enqueue_stencil<ed::hsplit>(c, Wh, tb0, nb, c->stream, launch);
       a.g.wsh = c->wsh;
+      if (h16_unit(c)) {
+        a.unit = 1;
+        a.dplan = c->dplan_u.p;
+        a.splits = c->splits_u.p;
+      }
       launch(ED_KERNEL_CONTRACT, [&] {
         if (h16_groups(c) <= 1)
           ed::contract_mixed16_kernel<1><<<nblk, 64 * ed::NWAVE, 0, c->stream>>>(Wh, c->ph16.p, out, a);
@@ -1701,6 +1761,10 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
         if (value < 0 || value > 1) fail(ED_ERR_ARG, "item order must be 0 or 1");
         ctx->item_order = (int)value;
         break;
+      case ED_OPT_H16_UNIT:
+        if (value < 0 || value > 1) fail(ED_ERR_ARG, "ED_OPT_H16_UNIT must be 0 or 1");
+        ctx->h16_unit_opt = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown option %d", option);
     }
@@ -1791,8 +1855,10 @@ int ed_query(ed_context* ctx, int what, int64_t* value) {
           // 16 N-blocks x 12 MFMAs
           *value = what == ED_Q_MIXED_F32_FLOPS_PER_T
                        ? 0
-                       : ctx->npb * chunks2 * (32 * 4 * tw(ctx->h16_groups_sum) + tw(ctx->nd_tri) * 16 * 12) * 2 * 16 *
-                             16 * 32;
+                       : ctx->npb * chunks2 *
+                             (32 * 4 * tw(h16_unit(ctx) ? ctx->h16u_groups_sum : ctx->h16_groups_sum) +
+                              tw(ctx->nd_tri) * 16 * 12) *
+                             2 * 16 * 16 * 32;
           break;
         }
         if (what == ED_Q_MIXED_F32_FLOPS_PER_T)
@@ -1830,6 +1896,7 @@ int ed_get_option(ed_context* ctx, int option, int64_t* value) {
       case ED_OPT_ORTHO_GATE: *value = ctx->ortho_gate; break;
       case ED_OPT_STRIP: *value = ctx->strip; break;
       case ED_OPT_ITEM_ORDER: *value = ctx->item_order; break;
+      case ED_OPT_H16_UNIT: *value = ctx->h16_unit_opt; break;
       default: fail(ED_ERR_ARG, "unknown option %d", option);
     }
   });

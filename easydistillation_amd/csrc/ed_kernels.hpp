@@ -101,6 +101,7 @@ struct Geom {
   int V3, V3P, Ne, NeP;
   int nW;
   int wsh;  // scale exponent of an hsplit workspace (0 otherwise)
+  int rot;  // 1: the hop stores i times each child (a level-k slot holds i^k W)
 };
 
 // element (n, x, c, e) of one timeslice's W block
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
         re -= v.x * ab[b].x + v.y * ab[b].y;
         im -= v.x * ab[b].y - v.y * ab[b].x;
       }
-      return wput<c_t<T>>(re, im, g.wsh);
+      return g.rot ? wput<c_t<T>>(-im, re, g.wsh) : wput<c_t<T>>(re, im, g.wsh);
     };
     if constexpr (sizeof(c_t<T>) == 8) {
       // 8-byte elements: each lane takes the eigenvector pair (2 l, 2 l + 1),
@@ -341,7 +342,7 @@ struct DPlan {
   int mirror;       // rev(d) = d: only tiles te <= tf run; tile (te, tf) also
                     // writes the transposed tile (tf, te) through the partner rule
   int item0;        // first work item of this d within a timeslice
-  int pad;
+  int rot;          // the FP16 stage-1 kernel multiplies its result by i^rot
 };
 
 // The diagonal tile of a self-partnered derivative computes every (e, f)
@@ -369,6 +370,7 @@ struct ContractArgs {
   const double* psum;    // [npb*16] sum_x Phi_b(x): V3 when b = 0 mod L, else 0
   int strip;             // e-tile rows per strip of the tile order (0: the kernel's default)
   int order;             // single-lattice item order: 0 contiguous ranges per XCD, 1 round robin
+  int unit;              // FP16 stage-1 kernel: the unit plan (every coefficient +1, results times i^rot)
 };
 
 // Tile k of one (timeslice, derivative, momentum block): all nte x ntf tiles,
@@ -768,17 +770,21 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // Three-term split of two FP32 values: hi = fp16(x) (round to nearest even,
-// packed), lo = fp16(x - hi) with the residual formed exactly inside
-// v_fma_mix from the FP16 hi and the FP32 x.
-__device__ __forceinline__ void split16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+// packed), lo = fp16(x - hi) with the residual formed exactly inside one
+// v_fma_mix per value (fma(hi, m1, x) with m1 = -1 a runtime value, so the
+// compiler keeps the fused form instead of folding it to a subtraction).
+// Plain code, not inline assembly: the results feed matrix instructions,
+// and the compiler inserts the wait states a vector write before a matrix
+// read needs only for instructions it sees.
+__device__ __forceinline__ void split16x2(float x0, float x1, float m1, uint32_t& hi, uint32_t& lo) {
   typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-  const h2 h = {(_Float16)x0, (_Float16)x1};
-  hi = __builtin_bit_cast(uint32_t, h);
-  uint32_t l;
-  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(x0));
-  asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hi), "v"(x1));
-  lo = l;
+  const _Float16 h0 = (_Float16)x0, h1 = (_Float16)x1;
+  hi = __builtin_bit_cast(uint32_t, h2{h0, h1});
+  lo = __builtin_bit_cast(uint32_t, h2{(_Float16)__builtin_fmaf((float)h0, m1, x0), (_Float16)__builtin_fmaf((float)h1, m1, x1)});
 }
+
+// -1.0f from a kernel argument (Ne > 0): a value the compiler cannot fold
+__device__ __forceinline__ float runtime_m1(const Geom& g) { return g.Ne > 0 ? -1.f : 1.f; }
 
 template <int NKS>
 __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W, const half8* __restrict__ ph16,
@@ -867,6 +873,7 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
   // stage 2 of one chunk for N-block b.  Padded sites hold Q = 0 and Phi = 0:
   // their (0 - c) sq terms meet a zero phase and only enter the p = 0 sum,
   // which the epilogue corrects with V3P in place of V3.
+  const float m1 = runtime_m1(a.g);
   auto stage2 = [&](int b, int c0, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -884,8 +891,8 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
         zsr += sr[h];
         zsi += si[h];
       }
-      split16x2(sr[0], sr[1], qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
@@ -1326,6 +1333,7 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
   // their (0 - c) sq terms meet a zero phase).  qb points at this lane's
   // first site (8 g4) of the chunk, row 4 cw, column l16: the site (+QSITE
   // float2) and row (+16) offsets are immediates.
+  const float m1 = runtime_m1(a.g);
   auto stage2 = [&](int b, const float2* qb, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -1338,8 +1346,8 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
         sr[h] = fmaf(q.x, sq, csr);
         si[h] = fmaf(q.y, sq, csi);
       }
-      split16x2(sr[0], sr[1], qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});

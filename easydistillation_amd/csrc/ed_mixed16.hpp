@@ -32,33 +32,59 @@ namespace ed {
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 
 // c (l.x, l.y) and c (-l.y, l.x) of a packed FP16 pair: one packed multiply
-// each, the swap done by operand selection
+// each (the swap folds into its operand selection).  Plain vector code, not
+// inline assembly: these registers feed and follow matrix instructions, and
+// only instructions the compiler sees get the wait states that a vector
+// write before a matrix read, or after a matrix read of an accumulator,
+// requires.
 __device__ __forceinline__ uint32_t pk_scale(uint32_t l, uint32_t c2) {
-  uint32_t r;
-  asm("v_pk_mul_f16 %0, %1, %2" : "=v"(r) : "v"(l), "v"(c2));
-  return r;
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(h2v, l) * __builtin_bit_cast(h2v, c2));
 }
 __device__ __forceinline__ uint32_t pk_scale_swap(uint32_t l, uint32_t cm2) {
-  uint32_t r;
-  asm("v_pk_mul_f16 %0, %1, %2 op_sel:[1,0] op_sel_hi:[0,1]" : "=v"(r) : "v"(l), "v"(cm2));
-  return r;
+  const h2v v = __builtin_bit_cast(h2v, l);
+  return __builtin_bit_cast(uint32_t, h2v{v.y, v.x} * __builtin_bit_cast(h2v, cm2));
 }
 
-// (v.y, v.x): one 64-bit move with operand selection
-__device__ __forceinline__ uint64_t swap_pair(uint64_t v) {
-  uint64_t r;
-  asm("v_pk_mov_b32 %0, %1, %1 op_sel:[1,0]" : "=v"(r) : "v"(v));
-  return r;
+// (v.y, v.x) of a 64-bit pair
+__device__ __forceinline__ uint64_t swap_pair(uint64_t v) { return (v >> 32) | (v << 32); }
+
+// (re, im) times i^k (k uniform; exact)
+__device__ __forceinline__ void rot_i(int k, double& re, double& im) {
+  if (k & 1) {
+    const double t = re;
+    re = -im;
+    im = t;
+  }
+  if (k & 2) {
+    re = -re;
+    im = -im;
+  }
 }
 typedef uint64_t u2l __attribute__((ext_vector_type(2)));
 
-// The Q tiles live in LDS as (re, im) float pairs; pair (buf, site, col)
-// starts at float qoff(buf, site, col).
-__device__ __forceinline__ int qoff(int buf, int site, int col) { return 2 * ((buf * SC2 + site) * QSITE + col); }
-constexpr int QFLOATS = 2 * 2 * SC2 * QSITE + 2;
+// LDS layout of the Q tiles.  A site holds 16 rows e of 16 columns f in 512
+// floats, real and imaginary parts in separate 16-float slots: row e = 4 G + r
+// puts its real part in slot q and its imaginary part in slot 4 + q of the
+// 128-float block G, q = r ^ G ^ (site / 8) (mod 4).  The imaginary part is
+// always 64 floats after the real part and sites are 512 floats apart, so
+// every access of a lane is a fixed per-lane base plus immediate offsets (one
+// ds_write2 / ds_read2 per value pair, no register moves or address
+// arithmetic).  Banks (64 of 4 bytes): a producer store (one site, lanes:
+// G = g4, column l16) and a consumer load (one row, lanes: site 8 g4 + k,
+// column l16) both put lane group g4 in its own bank quarter.
+constexpr int QS16 = 16 * 32;
+__device__ __forceinline__ int qre(int buf, int site, int row, int col) {
+  return (buf * SC2 + site) * QS16 + 128 * (row >> 2) + 16 * (((row & 3) ^ (row >> 2) ^ (site >> 3)) & 3) + col;
+}
+constexpr int QIM = 64;  // imaginary part: qre(...) + QIM
+constexpr int QFLOATS = 2 * SC2 * QS16 + 2;
+static_assert(NPROD * 16 * 16 * (int)sizeof(double2) <= (int)sizeof(float) * QFLOATS,
+              "the producers' zero-momentum partial sums fit in the Q buffers");
 
-// Producer loop for exactly NG k-groups (8 split-colour pairs each).
-template <int NG>
+// Producer loop for exactly NG k-groups (8 split-colour pairs each).  UNIT:
+// every split coefficient is +1 (the unit plan), so A is the loaded L itself
+// and only the imaginary operand i L costs a packed multiply per register.
+template <int NG, bool UNIT>
 __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, const ContractArgs& a, const DPlan& dp,
                                                  int tb, int E0, int F0, float* __restrict__ Qf,
                                                  float (*wmax)[NWAVE]) {
@@ -93,6 +119,7 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
       cim[m][h] = __builtin_bit_cast(uint32_t, h2v{-cf, cf});
     }
   }
+  const uint32_t cunit = __builtin_bit_cast(uint32_t, h2v{(_Float16)-1.f, (_Float16)1.f});
   uint2 lv[NG][2][SPP], rv[NG][2][SPP];
   auto load = [&](int m, int i, int c0) {
     const int xo = (c0 + SPP * wave + i) * (int)site_b;  // uniform
@@ -103,66 +130,69 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
     }
   };
 
-  // zero-momentum sums of this wave's sites, rows 4 g4 + r, column l16, of
-  // Q - z0 with z0 this lane's mean over its sites of chunk 0 (added back in
-  // FP64 at the end), so the FP32 partial sums carry only fluctuations
+  // zero-momentum sums of this wave's sites, rows 4 g4 + r, column l16:
+  // FP32 over the wave's SPP sites of a chunk, then FP64
   double zr[4] = {0, 0, 0, 0}, zi[4] = {0, 0, 0, 0};
-  float z0r[4] = {0.f, 0.f, 0.f, 0.f}, z0i[4] = {0.f, 0.f, 0.f, 0.f};
 
-  auto stage1 = [&](int chunk, int buf, bool first) {
+  // one chunk: site i's MFMAs are issued before site i - 1's results are
+  // stored, so the stores, sums and maximum overlap the matrix pipe
+  auto stage1 = [&](int chunk, int buf) {
     const int c0n = min(chunk + 1, nchunk - 1) * SC2;  // the last chunk re-reads itself
-    acc_t qr[SPP], qi[SPP];
+    float mx = 0.f;
+    float sr[4], si[4];
+    // Q of site i to LDS, into the tile maximum and the zero-momentum sums
+    // (padded sites hold Q = 0 exactly)
+    auto finish = [&](int i, const acc_t& qr, const acc_t& qi) {
 #pragma unroll
-    for (int m = 0; m < NG; ++m) {
+      for (int r = 0; r < 4; ++r) {
+        const int o = qre(buf, SPP * wave + i, mfma<float>::row(g4, r), l16);
+        Qf[o] = qr[r];
+        Qf[o + QIM] = qi[r];
+        if (i == 0) {
+          sr[r] = qr[r];
+          si[r] = qi[r];
+        } else {
+          sr[r] += qr[r];
+          si[r] += qi[r];
+        }
+      }
+      mx = fmaxf(fmaxf(mx, fabsf(qr[0])), fabsf(qi[0]));
+      mx = fmaxf(fmaxf(mx, fabsf(qr[1])), fabsf(qi[1]));
+      mx = fmaxf(fmaxf(mx, fabsf(qr[2])), fabsf(qi[2]));
+      mx = fmaxf(fmaxf(mx, fabsf(qr[3])), fabsf(qi[3]));
+    };
+    acc_t pr = zero, pi = zero;
 #pragma unroll
-      for (int i = 0; i < SPP; ++i) {
+    for (int i = 0; i < SPP; ++i) {
+      acc_t qr = zero, qi = zero;
+#pragma unroll
+      for (int m = 0; m < NG; ++m) {
         const uint2 l0 = lv[m][0][i], l1 = lv[m][1][i], r0 = rv[m][0][i], r1 = rv[m][1][i];
-        const half8 Ar = __builtin_bit_cast(half8, u4v{pk_scale(l0.x, cre[m][0]), pk_scale(l0.y, cre[m][0]),
-                                                       pk_scale(l1.x, cre[m][1]), pk_scale(l1.y, cre[m][1])});
-        const half8 Ai = __builtin_bit_cast(half8, u4v{pk_scale_swap(l0.x, cim[m][0]), pk_scale_swap(l0.y, cim[m][0]),
-                                                       pk_scale_swap(l1.x, cim[m][1]), pk_scale_swap(l1.y, cim[m][1])});
+        half8 Ar, Ai;
+        if constexpr (UNIT) {
+          Ar = __builtin_bit_cast(half8, u4v{l0.x, l0.y, l1.x, l1.y});
+          Ai = __builtin_bit_cast(half8, u4v{pk_scale_swap(l0.x, cunit), pk_scale_swap(l0.y, cunit),
+                                             pk_scale_swap(l1.x, cunit), pk_scale_swap(l1.y, cunit)});
+        } else {
+          Ar = __builtin_bit_cast(half8, u4v{pk_scale(l0.x, cre[m][0]), pk_scale(l0.y, cre[m][0]),
+                                             pk_scale(l1.x, cre[m][1]), pk_scale(l1.y, cre[m][1])});
+          Ai = __builtin_bit_cast(half8, u4v{pk_scale_swap(l0.x, cim[m][0]), pk_scale_swap(l0.y, cim[m][0]),
+                                             pk_scale_swap(l1.x, cim[m][1]), pk_scale_swap(l1.y, cim[m][1])});
+        }
         const half8 B1 = __builtin_bit_cast(half8, u4v{r0.x, r0.y, r1.x, r1.y});
         const half8 B2 = __builtin_bit_cast(
             half8, u2l{swap_pair(__builtin_bit_cast(uint64_t, r0)), swap_pair(__builtin_bit_cast(uint64_t, r1))});
-        qr[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B1, m ? qr[i] : zero, 0, 0, 0);
-        qi[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B1, m ? qi[i] : zero, 0, 0, 0);
-        qr[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B2, qr[i], 0, 0, 0);
-        qi[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B2, qi[i], 0, 0, 0);
+        qr = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B1, qr, 0, 0, 0);
+        qi = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B1, qi, 0, 0, 0);
+        qr = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ar, B2, qr, 0, 0, 0);
+        qi = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ai, B2, qi, 0, 0, 0);
         load(m, i, c0n);
       }
+      if (i > 0) finish(i - 1, pr, pi);
+      pr = qr;
+      pi = qi;
     }
-    // Q to LDS, the tile maximum and the zero-momentum sums (padded sites
-    // hold Q = 0 exactly)
-    if (first) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float ar = 0.f, ai = 0.f;
-#pragma unroll
-        for (int i = 0; i < SPP; ++i) {
-          ar += qr[i][r];
-          ai += qi[i][r];
-        }
-        z0r[r] = ar * (1.f / SPP);
-        z0i[r] = ai * (1.f / SPP);
-      }
-    }
-    float mx = 0.f;
-    float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < SPP; ++i) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int o = qoff(buf, SPP * wave + i, mfma<float>::row(g4, r) * 16 + l16);
-        Qf[o] = qr[i][r];
-        Qf[o + 1] = qi[i][r];
-        sr[r] += qr[i][r] - z0r[r];
-        si[r] += qi[i][r] - z0i[r];
-      }
-      mx = fmaxf(fmaxf(mx, fabsf(qr[i][0])), fabsf(qi[i][0]));
-      mx = fmaxf(fmaxf(mx, fabsf(qr[i][1])), fabsf(qi[i][1]));
-      mx = fmaxf(fmaxf(mx, fabsf(qr[i][2])), fabsf(qi[i][2]));
-      mx = fmaxf(fmaxf(mx, fabsf(qr[i][3])), fabsf(qi[i][3]));
-    }
+    finish(SPP - 1, pr, pi);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       zr[r] += (double)sr[r];
@@ -174,25 +204,22 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
 
   __builtin_amdgcn_s_setprio(2);
 #pragma unroll
-  for (int m = 0; m < NG; ++m) {
+  for (int i = 0; i < SPP; ++i) {
 #pragma unroll
-    for (int i = 0; i < SPP; ++i) load(m, i, 0);
+    for (int m = 0; m < NG; ++m) load(m, i, 0);
   }
-  stage1(0, 0, true);
+  stage1(0, 0);
   __syncthreads();
   for (int chunk = 0; chunk + 1 < nchunk; ++chunk) {
-    stage1(chunk + 1, (chunk + 1) & 1, false);
+    stage1(chunk + 1, (chunk + 1) & 1);
     __syncthreads();
   }
   __syncthreads();  // matches the consumers' last chunk
   // every consumer has read its last chunk: LDS buffer 0 takes the four
   // producers' zero-momentum partial sums, rows x columns, as doubles
   double2* zs = (double2*)(void*)Qf;
-  const double nsum = (double)SPP * nchunk;  // sites this wave summed, padded ones (Q = 0) included
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-    zs[(wave * 16 + mfma<float>::row(g4, r)) * 16 + l16] =
-        make_double2(fma((double)z0r[r], nsum, zr[r]), fma((double)z0i[r], nsum, zi[r]));
+  for (int r = 0; r < 4; ++r) zs[(wave * 16 + mfma<float>::row(g4, r)) * 16 + l16] = make_double2(zr[r], zi[r]);
   __syncthreads();
 }
 
@@ -217,8 +244,14 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     // producer: one branch-free instance per k-group count (NKS groups of
     // eight split-colour pairs at most)
     const int ng = (3 * dp.nsplit + 7) >> 3;
-    if (NKS >= 2 && ng == 2) mixed16_producer<(NKS >= 2 ? 2 : 1)>(W, a, dp, tb, E0, F0, Qf, wmax);
-    else mixed16_producer<1>(W, a, dp, tb, E0, F0, Qf, wmax);
+    constexpr int NG2 = NKS >= 2 ? 2 : 1;
+    if (a.unit) {
+      if (NKS >= 2 && ng == 2) mixed16_producer<NG2, true>(W, a, dp, tb, E0, F0, Qf, wmax);
+      else mixed16_producer<1, true>(W, a, dp, tb, E0, F0, Qf, wmax);
+    } else {
+      if (NKS >= 2 && ng == 2) mixed16_producer<NG2, false>(W, a, dp, tb, E0, F0, Qf, wmax);
+      else mixed16_producer<1, false>(W, a, dp, tb, E0, F0, Qf, wmax);
+    }
     return;
   }
 
@@ -242,8 +275,9 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
 #pragma unroll
     for (int r = 0; r < 4; ++r) D1[b][r] = D2[b][r] = D3[b][r] = D4[b][r] = 0;
   }
-  // stage 2 of one chunk for row b; qb points at this lane's first site
-  // (8 g4) of the chunk, row 4 cw, column l16
+  // stage 2 of one chunk for row b; qb points at the real part of this
+  // lane's first site (8 g4) of the chunk, row NBC cw + b, column l16
+  const float m1 = runtime_m1(a.g);
   auto stage2 = [&](int b, const float* qb, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -252,12 +286,12 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
       float sr[2], si[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const float2 q = ((const float2*)qb)[(2 * v2 + h) * QSITE + b * 16];
-        sr[h] = fmaf(q.x, sq, csr);
-        si[h] = fmaf(q.y, sq, csi);
+        const float* q = qb + (2 * v2 + h) * QS16;
+        sr[h] = fmaf(q[0], sq, csr);
+        si[h] = fmaf(q[QIM], sq, csi);
       }
-      split16x2(sr[0], sr[1], qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
@@ -293,9 +327,9 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     for (int b = 0; b < NBC; ++b) {
       float sr = 0, si = 0;
       for (int xl = 0; xl < n0; ++xl) {
-        const int o = qoff(0, xl, (NBC * cw + b) * 16 + l16);
+        const int o = qre(0, xl, NBC * cw + b, l16);
         sr += Qf[o];
-        si += Qf[o + 1];
+        si += Qf[o + QIM];
       }
       cr[b] = sr / (float)n0;
       ci[b] = si / (float)n0;
@@ -309,10 +343,9 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     int ex = 0;
     if (tmax > 0.f) (void)frexpf(2.f * tmax, &ex);
     const float sq = ldexpf(1.f, 14 - ex);
-    const float* qb = Qf + qoff(buf, 8 * g4, NBC * cw * 16 + l16);
 #pragma unroll
     for (int b = 0; b < NBC; ++b) {
-      stage2(b, qb, sq, ex);
+      stage2(b, Qf + qre(buf, 8 * g4, NBC * cw + b, l16), sq, ex);
       __builtin_amdgcn_sched_barrier(0);
     }
     if (chunk + 1 < nchunk) load_phase(chunk + 1);
@@ -339,8 +372,10 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
       if (!ok || bi >= a.nbase) continue;
       const int ip = a.plus_idx[bi], im = a.minus_idx[bi];
       const double S = a.psum[bi], mr = (double)cr[b] * S, mi = (double)ci[b] * S;
-      const double epr = D1[b][r] - D2[b][r] + mr, epi = D3[b][r] + D4[b][r] + mi;
-      const double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
+      double epr = D1[b][r] - D2[b][r] + mr, epi = D3[b][r] + D4[b][r] + mi;
+      double emr = D1[b][r] + D2[b][r] + mr, emi = D4[b][r] - D3[b][r] + mi;
+      rot_i(dp.rot, epr, epi);
+      rot_i(dp.rot, emr, emi);
       if (ip >= 0) out[(tbase + (size_t)dp.d * P + ip) * plane + (size_t)e * Ne + f] = make_double2(epr * wef, epi * wef);
       if (im >= 0) out[(tbase + (size_t)dp.d * P + im) * plane + (size_t)e * Ne + f] = make_double2(emr * wef, emi * wef);
       if (partner_writes(dp, te, tf)) {
@@ -357,7 +392,8 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     const int row = NBC * cw + b;
     const double2 z0 = zs[(0 * 16 + row) * 16 + l16], z1 = zs[(1 * 16 + row) * 16 + l16];
     const double2 z2 = zs[(2 * 16 + row) * 16 + l16], z3 = zs[(3 * 16 + row) * 16 + l16];
-    const double zre = (z0.x + z1.x) + (z2.x + z3.x), zim = (z0.y + z1.y) + (z2.y + z3.y);
+    double zre = (z0.x + z1.x) + (z2.x + z3.x), zim = (z0.y + z1.y) + (z2.y + z3.y);
+    rot_i(dp.rot, zre, zim);
     if (ok && g4 == 0 && pb == 0 && a.zero_idx >= 0) {
       out[(tbase + (size_t)dp.d * P + a.zero_idx) * plane + (size_t)e * Ne + f] = make_double2(zre * wef, zim * wef);
       if (partner_writes(dp, te, tf)) {

@@ -191,7 +191,11 @@ int ed_synchronize(ed_context* ctx);
  *                         4 for the mixed kernels, 2 for the FP64 / FP32 kernel).  Chosen by
  *                         ed_tune.
  *   ED_OPT_ITEM_ORDER     contraction work items over the 8 XCDs: 0 (default) contiguous ranges
- *                         per XCD, 1 round robin.  Chosen by ed_tune. */
+ *                         per XCD, 1 round robin.  Chosen by ed_tune.
+ *   ED_OPT_H16_UNIT       1 (default): ED_OPT_MIXED_WS 2 runs the unit plan where it applies
+ *                         (every split coefficient +1: the workspace stores each derivative
+ *                         level k times i^k, splits of multiplicity m enter m times, results
+ *                         are multiplied back by i^{|d|}); 0: coefficients scale the operands. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
@@ -203,7 +207,8 @@ enum ed_option {
   ED_OPT_WIDE_WS = 8,
   ED_OPT_ORTHO_GATE = 9,
   ED_OPT_STRIP = 10,
-  ED_OPT_ITEM_ORDER = 11
+  ED_OPT_ITEM_ORDER = 11,
+  ED_OPT_H16_UNIT = 12
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 int ed_get_option(ed_context* ctx, int option, int64_t* value);
