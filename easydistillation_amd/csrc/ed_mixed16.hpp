@@ -30,6 +30,7 @@
 namespace ed {
 
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 // c (l.x, l.y) and c (-l.y, l.x) of a packed FP16 pair: one packed multiply
 // each (the swap folds into its operand selection).  Plain vector code, not
@@ -119,7 +120,10 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
       cim[m][h] = __builtin_bit_cast(uint32_t, h2v{-cf, cf});
     }
   }
-  const uint32_t cunit = __builtin_bit_cast(uint32_t, h2v{(_Float16)-1.f, (_Float16)1.f});
+  // (-1, 1) from a runtime value: as a compile-time constant the compiler
+  // rewrites the swap-and-multiply into a multiply and a separate rotate
+  const _Float16 m1h = (_Float16)runtime_m1(g);
+  const h8v cunit = {m1h, -m1h, m1h, -m1h, m1h, -m1h, m1h, -m1h};
   uint2 lv[NG][2][SPP], rv[NG][2][SPP];
   auto load = [&](int m, int i, int c0) {
     const int xo = (c0 + SPP * wave + i) * (int)site_b;  // uniform
@@ -170,9 +174,11 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
         const uint2 l0 = lv[m][0][i], l1 = lv[m][1][i], r0 = rv[m][0][i], r1 = rv[m][1][i];
         half8 Ar, Ai;
         if constexpr (UNIT) {
-          Ar = __builtin_bit_cast(half8, u4v{l0.x, l0.y, l1.x, l1.y});
-          Ai = __builtin_bit_cast(half8, u4v{pk_scale_swap(l0.x, cunit), pk_scale_swap(l0.y, cunit),
-                                             pk_scale_swap(l1.x, cunit), pk_scale_swap(l1.y, cunit)});
+          // i L = (-Im, Re) per pair: one swap-and-multiply over the whole
+          // operand (a packed multiply per register)
+          const h8v L = __builtin_bit_cast(h8v, u4v{l0.x, l0.y, l1.x, l1.y});
+          Ar = __builtin_bit_cast(half8, L);
+          Ai = __builtin_bit_cast(half8, __builtin_shufflevector(L, L, 1, 0, 3, 2, 5, 4, 7, 6) * cunit);
         } else {
           Ar = __builtin_bit_cast(half8, u4v{pk_scale(l0.x, cre[m][0]), pk_scale(l0.y, cre[m][0]),
                                              pk_scale(l1.x, cre[m][1]), pk_scale(l1.y, cre[m][1])});
@@ -191,6 +197,10 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
       if (i > 0) finish(i - 1, pr, pi);
       pr = qr;
       pi = qi;
+      // keep each site's operand preparation with its own matrix
+      // instructions: hoisted to the top of the chunk, it would wait there
+      // for loads issued late in the previous chunk
+      __builtin_amdgcn_sched_barrier(0);
     }
     finish(SPP - 1, pr, pi);
 #pragma unroll
@@ -278,6 +288,7 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   // stage 2 of one chunk for row b; qb points at the real part of this
   // lane's first site (8 g4) of the chunk, row NBC cw + b, column l16
   const float m1 = runtime_m1(a.g);
+  const uint32_t z0 = runtime_z0(a.g);
   auto stage2 = [&](int b, const float* qb, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -290,8 +301,8 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
         sr[h] = fmaf(q[0], sq, csr);
         si[h] = fmaf(q[QIM], sq, csi);
       }
-      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, z0, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, z0, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
