@@ -40,12 +40,12 @@ TRAFFIC_MEASURED = {
     ("C", "fp64", None, 2, 0): (434679524.2 * 2.0 + 2751779.8) * 1024,  # contract_kernel<double, 3>, 8 timeslices
     # contract_mixed16_kernel<2> (the tuned mixed kernel), 8 timeslices:
     # profiles/elemental_32c64_nev200_tuned_kernel_trace.md
-    ("C", "mixed", 5, 4, 1): (178699000.0 * 2.0 + 3328990.0) * 1024,
+    ("C", "mixed", 5, 2, 1): (180556618.2 * 2.0 + 3430244.9) * 1024,
 }
 TRAFFIC_SOURCE = {
     ("C", "mixed", 3, 4, 0): "profiles/elemental_32c64_nev200_hbm_traffic.md",
     ("C", "fp64", None, 2, 0): "profiles/elemental_32c64_nev200_hbm_traffic.md",
-    ("C", "mixed", 5, 4, 1): "profiles/elemental_32c64_nev200_tuned_kernel_trace.md",
+    ("C", "mixed", 5, 2, 1): "profiles/elemental_32c64_nev200_tuned_kernel_trace.md",
 }
 # the library's ED_Q_MIXED_KERNEL answer -> the kernel the mixed form runs
 MIXED_KERNEL = {
