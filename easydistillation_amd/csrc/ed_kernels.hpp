@@ -62,9 +62,13 @@ template <> __device__ __forceinline__ float2 wput<float2>(double re, double im,
   return make_float2((float)re, (float)im);
 }
 template <> __device__ __forceinline__ uint2 wput<uint2>(double re, double im, int wsh) {
-  // the residual is formed from the FP32 rounding of w 2^wsh (2^-24 of it,
-  // below the pair's 2^-22), exactly inside v_fma_mix
-  const float xr = (float)ldexp(re, wsh), xi = (float)ldexp(im, wsh);
+  // The residual is formed from the FP32 rounding of w 2^wsh (2^-24 of it,
+  // below the pair's 2^-22), exactly inside v_fma_mix.  With contraction
+  // allowed the compiler folds the two roundings double -> float -> half
+  // into one double -> half conversion, which has no instruction and runs as
+  // ~40 integer operations: keep them apart.
+#pragma clang fp contract(off)
+  const float xr = ldexpf((float)re, wsh), xi = ldexpf((float)im, wsh);
   const h2v h = {(_Float16)xr, (_Float16)xi};
   const uint32_t hu = __builtin_bit_cast(uint32_t, h);
   uint32_t l;
@@ -201,19 +205,27 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
       lb[k] = ub[k];
     }
     const int child = child0 + 3 * pl + mu;
+    // an hsplit workspace holds w 2^wsh; the hop is linear, so it reads and
+    // writes those scaled values as they are (scale exponent 0 below)
     // (U_mu(x) af - U_mu(x - mu)^dag ab)_a for one eigenvector
     auto hop = [&](const double2* af, const double2* ab, int a) {
+      // one FMA per real product (written out: the compound expressions
+      // compile to a multiply and an add per term)
       double re = 0, im = 0;
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         const double2 u = lf[a * 3 + b];
-        re += u.x * af[b].x - u.y * af[b].y;
-        im += u.x * af[b].y + u.y * af[b].x;
+        re = fma(u.x, af[b].x, re);
+        re = fma(-u.y, af[b].y, re);
+        im = fma(u.x, af[b].y, im);
+        im = fma(u.y, af[b].x, im);
         const double2 v = lb[b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
-        re -= v.x * ab[b].x + v.y * ab[b].y;
-        im -= v.x * ab[b].y - v.y * ab[b].x;
+        re = fma(-v.x, ab[b].x, re);
+        re = fma(-v.y, ab[b].y, re);
+        im = fma(-v.x, ab[b].y, im);
+        im = fma(v.y, ab[b].x, im);
       }
-      return g.rot ? wput<c_t<T>>(-im, re, g.wsh) : wput<c_t<T>>(re, im, g.wsh);
+      return g.rot ? wput<c_t<T>>(-im, re, 0) : wput<c_t<T>>(re, im, 0);
     };
     if constexpr (sizeof(c_t<T>) == 8) {
       // 8-byte elements: each lane takes the eigenvector pair (2 l, 2 l + 1),
@@ -227,10 +239,10 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
         for (int b = 0; b < 3; ++b) {
           const u4 pf = *(const u4*)&Wt[widx(g, parent, sf[mu], b, e)];
           const u4 pb = *(const u4*)&Wt[widx(g, parent, sb[mu], b, e)];
-          af[0][b] = wget(__builtin_bit_cast(c_t<T>, u2{pf.x, pf.y}), g.wsh);
-          af[1][b] = wget(__builtin_bit_cast(c_t<T>, u2{pf.z, pf.w}), g.wsh);
-          ab[0][b] = wget(__builtin_bit_cast(c_t<T>, u2{pb.x, pb.y}), g.wsh);
-          ab[1][b] = wget(__builtin_bit_cast(c_t<T>, u2{pb.z, pb.w}), g.wsh);
+          af[0][b] = wget(__builtin_bit_cast(c_t<T>, u2{pf.x, pf.y}), 0);
+          af[1][b] = wget(__builtin_bit_cast(c_t<T>, u2{pf.z, pf.w}), 0);
+          ab[0][b] = wget(__builtin_bit_cast(c_t<T>, u2{pb.x, pb.y}), 0);
+          ab[1][b] = wget(__builtin_bit_cast(c_t<T>, u2{pb.z, pb.w}), 0);
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -244,8 +256,8 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
         double2 af[3], ab[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-          af[b] = wget(Wt[widx(g, parent, sf[mu], b, e)], g.wsh);
-          ab[b] = wget(Wt[widx(g, parent, sb[mu], b, e)], g.wsh);
+          af[b] = wget(Wt[widx(g, parent, sf[mu], b, e)], 0);
+          ab[b] = wget(Wt[widx(g, parent, sb[mu], b, e)], 0);
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) Wo[widx(g, child, s, a, e)] = hop(af, ab, a);
@@ -288,6 +300,8 @@ __global__ __launch_bounds__(256) void disp_kernel(c_t<T>* __restrict__ W, const
     sb[2] = r + (z == 0 ? g.Lz - 1 : z - 1) * g.Lx * g.Ly;
   }
   c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;
+  // an hsplit workspace holds w 2^wsh; transport and mean are linear, so
+  // they read and write those scaled values as they are (exponent 0 below)
   double mr[3] = {0, 0, 0}, mi[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -298,8 +312,8 @@ __global__ __launch_bounds__(256) void disp_kernel(c_t<T>* __restrict__ W, const
     double2 pf[3], pb[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      pf[b] = wget(Wt[widx(g, slot_f, sf[d], b, e)], g.wsh);
-      pb[b] = wget(Wt[widx(g, slot_b, sb[d], b, e)], g.wsh);
+      pf[b] = wget(Wt[widx(g, slot_f, sf[d], b, e)], 0);
+      pb[b] = wget(Wt[widx(g, slot_b, sb[d], b, e)], 0);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -314,15 +328,15 @@ __global__ __launch_bounds__(256) void disp_kernel(c_t<T>* __restrict__ W, const
         bi += v.x * pb[b].y - v.y * pb[b].x;
       }
       if (dst0 >= 0) {
-        Wt[widx(g, dst0 + d, s, a, e)] = wput<c_t<T>>(fr, fi, g.wsh);
-        Wt[widx(g, dst0 + 3 + d, s, a, e)] = wput<c_t<T>>(br, bi, g.wsh);
+        Wt[widx(g, dst0 + d, s, a, e)] = wput<c_t<T>>(fr, fi, 0);
+        Wt[widx(g, dst0 + 3 + d, s, a, e)] = wput<c_t<T>>(br, bi, 0);
       }
       mr[a] += fr + br;
       mi[a] += fi + bi;
     }
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) Wt[widx(g, mslot, s, a, e)] = wput<c_t<T>>(mr[a] / 6.0, mi[a] / 6.0, g.wsh);
+  for (int a = 0; a < 3; ++a) Wt[widx(g, mslot, s, a, e)] = wput<c_t<T>>(mr[a] / 6.0, mi[a] / 6.0, 0);
 }
 
 // --------------------------------------------------------------------------
