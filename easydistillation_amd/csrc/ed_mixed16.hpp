@@ -236,7 +236,8 @@ __device__ __forceinline__ void mixed16_producer(const uint2* __restrict__ W, co
 template <int NKS>
 __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ W, const half8* __restrict__ ph16,
                                                       double2* __restrict__ out, const ContractArgs& a, const int w,
-                                                      float* __restrict__ Qf, float (*wmax)[NWAVE]) {
+                                                      float* __restrict__ Qf, float (*wmax)[NWAVE],
+                                                      half8 (*PHs)[4][64]) {
   typedef float4v acc_t;
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -269,15 +270,17 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   const int cw = wave - NPROD;
   const half8* PH = ph16 + (size_t)pb * nchunk * 4 * 16 * 4;
   const __amdgpu_buffer_rsrc_t pr_ = __builtin_amdgcn_make_buffer_rsrc((void*)PH, (short)0, nchunk * 4 * 16 * 4 * 16, 0x00020000);
+  // The chunk's phase operand, 4 parts (Re hi, Re lo, Im hi, Im lo) of 1 KB,
+  // crosses the load path once per workgroup: consumer cw loads part cw two
+  // chunks ahead, stores it to LDS one chunk ahead (double buffer PHs), and
+  // every consumer reads all four parts from LDS.  Lane (l16, g4) holds
+  // momentum l16, sites 8 g4 .. 8 g4 + 7 of a part.
+  static_assert(NBC == 4, "one phase part per consumer wave");
   const int phoff = (l16 * 4 + g4) * 16;
-  half8 prh, prl, pih, pil;
-  auto load_phase = [&](int chunk) {
-    const int co = chunk * 4 * 16 * 4 * 16;  // uniform
-    prh = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 0 * 1024, 0));
-    prl = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 1 * 1024, 0));
-    pih = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 2 * 1024, 0));
-    pil = __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, co + 3 * 1024, 0));
+  auto load_part = [&](int chunk) {
+    return __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(pr_, phoff, chunk * 4096 + cw * 1024, 0));
   };
+  half8 prh, prl, pih, pil;
   double D1[NBC][4], D2[NBC][4], D3[NBC][4], D4[NBC][4];
   float cr[NBC], ci[NBC];
 #pragma unroll
@@ -330,8 +333,10 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
     }
   };
 
-  load_phase(0);
-  __syncthreads();
+  half8 nxt = load_part(0);
+  PHs[0][cw][lane] = nxt;
+  if (nchunk > 1) nxt = load_part(1);
+  __syncthreads();  // chunk 0: Q and phase in LDS
   {
     const int n0 = min(SC2, g.V3);
 #pragma unroll
@@ -348,6 +353,10 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   }
   for (int chunk = 0; chunk < nchunk; ++chunk) {
     const int buf = chunk & 1;
+    prh = PHs[buf][0][lane];
+    prl = PHs[buf][1][lane];
+    pih = PHs[buf][2][lane];
+    pil = PHs[buf][3][lane];
     float tmax = 0.f;
 #pragma unroll
     for (int v = 0; v < NPROD; ++v) tmax = fmaxf(tmax, wmax[buf][v]);
@@ -359,7 +368,9 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
       stage2(b, Qf + qre(buf, 8 * g4, NBC * cw + b, l16), sq, ex);
       __builtin_amdgcn_sched_barrier(0);
     }
-    if (chunk + 1 < nchunk) load_phase(chunk + 1);
+    // buffer (chunk + 1) & 1 was last read in the previous chunk
+    if (chunk + 1 < nchunk) PHs[(chunk + 1) & 1][cw][lane] = nxt;
+    if (chunk + 2 < nchunk) nxt = load_part(chunk + 2);
     __syncthreads();
   }
   __syncthreads();  // the producers' zero-momentum partial sums are in LDS
@@ -423,9 +434,10 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed16_kernel(const u
                                                                           ContractArgs a) {
   __shared__ __align__(16) float Qf[QFLOATS];
   __shared__ float wmax[2][NWAVE];
+  __shared__ half8 PHs[2][4][64];
   const int w = item_index(a, blockIdx.x, gridDim.x);
   if (w >= a.nwork) return;
-  contract_mixed16_body<NKS>(W, ph16, out, a, w, Qf, wmax);
+  contract_mixed16_body<NKS>(W, ph16, out, a, w, Qf, wmax, PHs);
 }
 
 }  // namespace ed
