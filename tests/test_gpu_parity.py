@@ -307,6 +307,34 @@ def test_mixed_warp_specialized_matches_uniform(case):
     assert max(errs.values()) <= MIXED_TOL
 
 
+@pytest.mark.parametrize("case", [0, 4])
+def test_fp16_split_sum_unit_plan(case):
+    """The FP16 split-sum kernel (ED_OPT_MIXED_WS 2) on its unit plan (the
+    workspace holds i^k times each level-k derivative slot, every split
+    coefficient becomes +1 with its multiplicity as repeated terms, results
+    are multiplied by i^|d|) and on the scaled plan (coefficients multiply the
+    operands): each against the oracle, and against each other."""
+    latt, Ne, nn, moms, links = WS_CASES[case]
+    U, V = S.make_inputs_np(latt, Ne, links, seed=90 + case)
+    say(f"unit plan: latt={latt} Ne={Ne} nabla={nn} P={len(moms)} links={links}")
+    outs = {}
+    for unit in (1, 0):
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode="mixed")
+        g.set_option(_native.ED_OPT_MIXED_STAGE2, 1)
+        g.set_option(_native.ED_OPT_MIXED_WS, 2)
+        g.set_option(_native.ED_OPT_H16_UNIT, unit)
+        assert g.context.get_option(_native.ED_OPT_H16_UNIT) == unit
+        g.load_arrays(U, V)
+        assert g.context.query(_native.ED_Q_MIXED_KERNEL) == 5
+        outs[unit] = g.calc_all()
+    ref = O.elemental(V, U, latt, nn, moms)
+    errs = {unit: blockwise(outs[unit], ref) for unit in outs}
+    d = blockwise(outs[1], outs[0])
+    say(f"  vs oracle: unit {errs[1]:.3e}, scaled {errs[0]:.3e}; unit vs scaled {d:.2e}")
+    assert max(errs.values()) <= MIXED_TOL
+    assert d <= 2 * MIXED_TOL
+
+
 def test_load_from_ildg_and_qdp_files(tmp_path):
     """load(key) through the reference's on-disk formats: ILDG/LIME links and
     QDP LazyDiskMapObj eigenvectors (elemental.py:102-105, preset.py:55-64,151-160)."""
