@@ -452,6 +452,9 @@ def main():
                     help="mixed form's FP16 stage-2 kernel: 1 warp-specialized with FP32 stage 1 (default), "
                     "2 warp-specialized with FP16 stage 1, 0 uniform waves")
     ap.add_argument("--tbatch", type=int, default=0, help="timeslices per contraction launch (0: the library's choice)")
+    ap.add_argument("--hop-lds", type=int, default=-1, choices=(-1, 0, 1),
+                    help="derivative hops: -1 LDS-staged for 16-byte workspace elements (library default), "
+                    "1 LDS-staged for every form, 0 one wave per site")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (easydistillation_amd.tune) and run the library defaults / --mixed-ws")
     ap.add_argument("--tune-cache", default=None, help="JSON table of tuned shapes (default: the tuner's own)")
@@ -522,6 +525,7 @@ def main():
     for form in forms:
         g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode=form)
         g.set_option(_native.ED_OPT_MIXED_WS, args.mixed_ws)
+        g.set_option(_native.ED_OPT_HOP_LDS, args.hop_lds)
         if args.tbatch > 0:
             g.set_option(_native.ED_OPT_TBATCH, args.tbatch)
         g.load_arrays(U, V)
@@ -581,6 +585,7 @@ def main():
             "mixed_kernel": mixed_kernel,
             "strip": g.context.get_option(_native.ED_OPT_STRIP),
             "item_order": g.context.get_option(_native.ED_OPT_ITEM_ORDER),
+            "hop_lds": g.context.get_option(_native.ED_OPT_HOP_LDS),
             "tuned": tuned,
         }
         log(f"{form}: {json.dumps(results[form])}")

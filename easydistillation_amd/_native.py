@@ -26,6 +26,7 @@ ED_OPT_ORTHO_GATE = 9
 ED_OPT_STRIP = 10
 ED_OPT_ITEM_ORDER = 11
 ED_OPT_H16_UNIT = 12
+ED_OPT_HOP_LDS = 13
 ED_KERNEL_STENCIL, ED_KERNEL_CONTRACT, ED_KERNEL_STAGE, ED_KERNEL_ADJOINT = 0, 1, 2, 3
 
 # Every exported symbol and its ctypes signature (restype, argtypes).

@@ -164,6 +164,7 @@ struct ed_context {
   DevBuf<SplitTerm> splits;
   // unit plan of the FP16 stage-1 kernel (build_plan): every coefficient +1
   int h16_unit_opt = 1;               // ED_OPT_H16_UNIT
+  int hop_lds = -1;                   // ED_OPT_HOP_LDS: the LDS-staged hop where the shape allows
   bool unit_ok = false;
   bool ws_rot = false;                // the unit plan needs slot n stored times i^|n|
   int nsplit_u_max = 1;               // largest split count of the unit plan
@@ -845,8 +846,18 @@ void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t 
     const int parent0 = level_start(k - 1), nparent = level_start(k) - parent0;
     const int child0 = level_start(k);
     launch(ED_KERNEL_STENCIL, [&] {
-      dim3 grid((c->V3 + 3) / 4, nparent, nb);
-      ed::hop3_kernel<T><<<grid, 256, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0);
+      constexpr int EC = ed::hl_ec<T>();
+      // LDS-staged by default for 16-byte elements only: for 8-byte ones the
+      // links re-read per eigenvector chunk cost more than the plane reuse saves
+      const bool lds = c->hop_lds > 0 || (c->hop_lds < 0 && sizeof(ed::c_t<T>) == 16);
+      if (lds && c->Lx % ed::HL_T == 0 && c->Ly % ed::HL_T == 0 && c->Lz >= 3 && c->NeP % EC == 0) {
+        dim3 grid((c->Lx / ed::HL_T) * (c->Ly / ed::HL_T), nparent * (c->NeP / EC), nb);
+        constexpr int NW = EC == 16 ? 16 : 8;  // one site per wave and iteration
+        ed::hop3_lds_kernel<T, NW><<<grid, 64 * NW, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0, nparent);
+      } else {
+        dim3 grid((c->V3 + 3) / 4, nparent, nb);
+        ed::hop3_kernel<T><<<grid, 256, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0);
+      }
     });
   }
 }
@@ -1765,6 +1776,10 @@ int ed_set_option(ed_context* ctx, int option, int64_t value) {
         if (value < 0 || value > 1) fail(ED_ERR_ARG, "ED_OPT_H16_UNIT must be 0 or 1");
         ctx->h16_unit_opt = (int)value;
         break;
+      case ED_OPT_HOP_LDS:
+        if (value < -1 || value > 1) fail(ED_ERR_ARG, "ED_OPT_HOP_LDS must be -1, 0 or 1");
+        ctx->hop_lds = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown option %d", option);
     }
@@ -1897,6 +1912,7 @@ int ed_get_option(ed_context* ctx, int option, int64_t* value) {
       case ED_OPT_STRIP: *value = ctx->strip; break;
       case ED_OPT_ITEM_ORDER: *value = ctx->item_order; break;
       case ED_OPT_H16_UNIT: *value = ctx->h16_unit_opt; break;
+      case ED_OPT_HOP_LDS: *value = ctx->hop_lds; break;
       default: fail(ED_ERR_ARG, "unknown option %d", option);
     }
   });

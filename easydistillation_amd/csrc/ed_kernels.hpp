@@ -267,6 +267,190 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
 }
 
 // --------------------------------------------------------------------------
+// LDS-staged hop (same outputs as hop3_kernel, bit for bit).  One workgroup:
+// one parent slot, HL_EC(T) eigenvectors (16 for 8-byte workspace elements,
+// 8 for 16-byte ones: 128 bytes per site and colour), one 8 x 8 tile of
+// (x, y) sites, marching over all z.  LDS holds
+//   * a ring of three planes (z - 1, z, z + 1) of the parent: 10 x 10 sites
+//     (the tile and a one-site halo) x 3 colours x HL_EC eigenvectors, and
+//   * the links the plane's 64 outputs need: U_x and U_y on the tile, U_x on
+//     the halo column x0 - 1 and U_y on the halo row y0 - 1 (backward links),
+//     and U_z on the tile for the planes z and z - 1 (two slots),
+// so every parent element crosses the load path once per workgroup plus the
+// x-y halo (100 / 64 of the tile), and every link once per eigenvector
+// chunk.  The next plane's elements and links are loaded into registers
+// while the current plane is computed and go to LDS between two barriers.
+// Lanes: (site of the iteration, eigenvector e), each computing the three
+// colours of its site's child elements from one read of the six neighbours
+// (the arithmetic of hop3_kernel, in its order); wave w of NW takes the
+// plane's sites SPI w ... SPI (w + 1) - 1, then SPI (w + NW) ...
+// Requires Lx, Ly multiples of 8, Lz >= 3, NeP a multiple of HL_EC(T).
+// grid (Lx/8 * Ly/8, nparent * NeP/HL_EC, TB), block 64 * NW
+// --------------------------------------------------------------------------
+constexpr int HL_T = 8, HL_H = HL_T + 2, HL_SITES = HL_H * HL_H;
+constexpr int HL_PSI_PIECES = HL_SITES * 3 * 8;      // 16-byte pieces of one plane (2400)
+constexpr int HL_LX = 0, HL_LY = 64 * 9, HL_LXH = 2 * 64 * 9, HL_LYH = HL_LXH + 8 * 9;
+constexpr int HL_LZ = HL_LYH + 8 * 9;                 // two slots of 64 * 9
+constexpr int HL_LINKS = HL_LZ + 2 * 64 * 9;          // double2 (2448, 39 KB)
+constexpr int HL_LINK_LOAD = HL_LZ + 64 * 9;          // pieces loaded per plane (1872)
+template <typename T> constexpr int hl_ec() { return 128 / (int)sizeof(c_t<T>); }
+
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void hop3_lds_kernel(c_t<T>* __restrict__ W, const double2* __restrict__ U,
+                                                              Geom g, int tb0, int parent0, int child0, int nparent) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  constexpr int NT = 64 * NW;       // threads
+  constexpr int EC = hl_ec<T>();
+  constexpr int SPI = 64 / EC;      // sites per wave iteration
+  constexpr int ITER = 64 / (NW * SPI);
+  constexpr int PR = (HL_PSI_PIECES + NT - 1) / NT, LR = (HL_LINK_LOAD + NT - 1) / NT;
+  static_assert(64 % (NW * SPI) == 0, "the waves' iterations cover the 64 tile sites");
+  __shared__ __align__(16) u4 psi[3][HL_PSI_PIECES];
+  __shared__ __align__(16) double2 lk[HL_LINKS];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ntx = g.Lx / HL_T;
+  const int x0 = (blockIdx.x % ntx) * HL_T, y0 = (blockIdx.x / ntx) * HL_T;
+  const int pl = blockIdx.y % nparent, ec = blockIdx.y / nparent;
+  const int e0 = ec * EC;
+  const int tb = blockIdx.z, t = tb0 + tb;
+  const int parent = parent0 + pl;
+  const c_t<T>* Wt = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;
+  c_t<T>* Wo = W + (size_t)tb * g.nW * (size_t)g.V3P * 3 * g.NeP;
+  auto wrap = [](int v, int L) { return v < 0 ? v + L : (v >= L ? v - L : v); };
+  auto link_ptr = [&](int mu, int x, int y, int z) {
+    return U + (((size_t)mu * g.Lt + t) * g.V3 + ((size_t)z * g.Ly + y) * g.Lx + x) * 9;
+  };
+
+  // register staging of one plane: its parent pieces and its links
+  u4 sp[PR], sl[LR];
+  auto fetch_psi = [&](int z) {
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+      // pieces past the plane's last re-read the last one (stored nowhere):
+      // every load is unconditional, so the staging stays in registers
+      const int k = min(tid + NT * r, HL_PSI_PIECES - 1);
+      const int sh = k / 24, rem = k - sh * 24, c = rem >> 3, pc = rem & 7;
+      const int hy = sh / HL_H, hx = sh - hy * HL_H;
+      const int x = wrap(x0 + hx - 1, g.Lx), y = wrap(y0 + hy - 1, g.Ly);
+      const int s = (z * g.Ly + y) * g.Lx + x;
+      sp[r] = ((const u4*)&Wt[widx(g, parent, s, c, e0)])[pc];
+    }
+  };
+  auto store_psi = [&](int slot) {
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+      const int k = tid + NT * r;
+      if (k < HL_PSI_PIECES) psi[slot][k] = sp[r];
+    }
+  };
+  // links of plane z: U_x, U_y on the tile, the two backward halo strips, U_z
+  auto fetch_links = [&](int z) {
+#pragma unroll
+    for (int r = 0; r < LR; ++r) {
+      // region of piece k: tile U_x (q < 64), tile U_y (< 128), halo column
+      // U_x (< 136), halo row U_y (< 144), tile U_z (< 208); pieces past the
+      // last re-read it (stored nowhere)
+      const int k = min(tid + NT * r, HL_LINK_LOAD - 1);
+      const int q = k / 9, pc = k - q * 9;
+      const int qt = q < 64 ? q : (q < 128 ? q - 64 : q - 144);  // tile site of the tile regions
+      const int mu = q < 64 ? 0 : (q < 128 ? 1 : (q < 136 ? 0 : (q < 144 ? 1 : 2)));
+      const int x = q < 128 || q >= 144 ? x0 + (qt & 7) : (q < 136 ? wrap(x0 - 1, g.Lx) : x0 + (q - 136));
+      const int y = q < 128 || q >= 144 ? y0 + (qt >> 3) : (q < 136 ? y0 + (q - 128) : wrap(y0 - 1, g.Ly));
+      sl[r] = ((const u4*)link_ptr(mu, x, y, z))[pc];
+    }
+  };
+  // to LDS; the plane's U_z goes to slot zslot
+  auto store_links = [&](int zslot) {
+#pragma unroll
+    for (int r = 0; r < LR; ++r) {
+      const int k = tid + NT * r;
+      if (k < HL_LINK_LOAD) lk[k < HL_LZ ? k : k + zslot * 64 * 9] = __builtin_bit_cast(double2, sl[r]);
+    }
+  };
+
+  // planes Lz - 1, 0, 1 in ring slots 2, 0, 1; links of plane 0, U_z of
+  // plane Lz - 1 in slot 1
+  fetch_psi(g.Lz - 1);
+  store_psi(2);
+  fetch_psi(0);
+  store_psi(0);
+  fetch_psi(1);
+  store_psi(1);
+  fetch_links(g.Lz - 1);
+  store_links(1);
+  __syncthreads();  // the slot-1 U_z is written; the other links of plane Lz - 1 are overwritten next
+  fetch_links(0);
+  store_links(0);
+  __syncthreads();
+
+  const int sub = lane / EC, e = lane - sub * EC;
+  const bool active = e0 + e < g.Ne;
+  for (int z = 0; z < g.Lz; ++z) {
+    const bool more = z + 1 < g.Lz;
+    if (more) {
+      fetch_psi(z + 2 < g.Lz ? z + 2 : z + 2 - g.Lz);
+      fetch_links(z + 1);
+    }
+    const int sz = z % 3, sf = (z + 1) % 3, sb = (z + 2) % 3;  // planes z, z + 1, z - 1
+    const int zc = z & 1;                                       // U_z slot of plane z
+    for (int it = 0; it < ITER; ++it) {
+      const int q = (wave + NW * it) * SPI + sub;  // tile site
+      const int qx = q & 7, qy = q >> 3, hx = qx + 1, hy = qy + 1;
+      const int s = (z * g.Ly + y0 + qy) * g.Lx + x0 + qx;
+#pragma unroll
+      for (int mu = 0; mu < 3; ++mu) {
+        // neighbour elements (halo coordinates, ring slot) and links
+        int fh, bh, fsl, bsl, lf0, lb0;
+        if (mu == 0) {
+          fh = hy * HL_H + hx + 1, bh = hy * HL_H + hx - 1, fsl = bsl = sz;
+          lf0 = HL_LX + q * 9, lb0 = qx > 0 ? HL_LX + (q - 1) * 9 : HL_LXH + qy * 9;
+        } else if (mu == 1) {
+          fh = (hy + 1) * HL_H + hx, bh = (hy - 1) * HL_H + hx, fsl = bsl = sz;
+          lf0 = HL_LY + q * 9, lb0 = qy > 0 ? HL_LY + (q - 8) * 9 : HL_LYH + qx * 9;
+        } else {
+          fh = bh = hy * HL_H + hx, fsl = sf, bsl = sb;
+          lf0 = HL_LZ + zc * 64 * 9 + q * 9, lb0 = HL_LZ + (zc ^ 1) * 64 * 9 + q * 9;
+        }
+        double2 af[3], ab[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          af[b] = wget(((const c_t<T>*)psi[fsl])[(fh * 3 + b) * EC + e], 0);
+          ab[b] = wget(((const c_t<T>*)psi[bsl])[(bh * 3 + b) * EC + e], 0);
+        }
+        const int child = child0 + 3 * pl + mu;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          // row a of U_mu(x) and column a of U_mu(x - mu): the same LDS
+          // address across the site's lanes (broadcast)
+          double re = 0, im = 0;
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            const double2 u = lk[lf0 + a * 3 + b];
+            re = fma(u.x, af[b].x, re);
+            re = fma(-u.y, af[b].y, re);
+            im = fma(u.x, af[b].y, im);
+            im = fma(u.y, af[b].x, im);
+            const double2 v = lk[lb0 + b * 3 + a];  // (U^dag)_{ab} = conj(U_{ba})
+            re = fma(-v.x, ab[b].x, re);
+            re = fma(-v.y, ab[b].y, re);
+            im = fma(-v.x, ab[b].y, im);
+            im = fma(v.y, ab[b].x, im);
+          }
+          if (active) Wo[widx(g, child, s, a, e0 + e)] = g.rot ? wput<c_t<T>>(-im, re, 0) : wput<c_t<T>>(re, im, 0);
+        }
+      }
+    }
+    __syncthreads();  // plane z - 1 and this plane's links are no longer read
+    if (more) {
+      store_psi(sb);
+      store_links(zc ^ 1);
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
 // Straight-path transport for displacement elementals
 // (lattice/generator/displacement_elemental.py:53-71): from the six paths of
 // length n - 1 (slots src0 + d forward, src0 + 3 + d backward; src0 < 0: all

@@ -195,7 +195,13 @@ int ed_synchronize(ed_context* ctx);
  *   ED_OPT_H16_UNIT       1 (default): ED_OPT_MIXED_WS 2 runs the unit plan where it applies
  *                         (every split coefficient +1: the workspace stores each derivative
  *                         level k times i^k, splits of multiplicity m enter m times, results
- *                         are multiplied back by i^{|d|}); 0: coefficients scale the operands. */
+ *                         are multiplied back by i^{|d|}); 0: coefficients scale the operands.
+ *   ED_OPT_HOP_LDS        derivative hops: 1 LDS-staged (an 8 x 8 (x, y) tile with its halo and
+ *                         the planes z - 1, z, z + 1 of 128 bytes of eigenvectors per site and
+ *                         colour in LDS, marching over z) where Lx and Ly are multiples of 8 and
+ *                         Lz >= 3; 0 one wave per site; -1 (default) LDS-staged for the FP64
+ *                         form's 16-byte workspace elements, one wave per site for the 8-byte
+ *                         ones of the mixed form.  Same results bit for bit. */
 enum ed_option {
   ED_OPT_HERMITIAN = 1,
   ED_OPT_TBATCH = 2,
@@ -208,7 +214,8 @@ enum ed_option {
   ED_OPT_ORTHO_GATE = 9,
   ED_OPT_STRIP = 10,
   ED_OPT_ITEM_ORDER = 11,
-  ED_OPT_H16_UNIT = 12
+  ED_OPT_H16_UNIT = 12,
+  ED_OPT_HOP_LDS = 13
 };
 int ed_set_option(ed_context* ctx, int option, int64_t value);
 int ed_get_option(ed_context* ctx, int option, int64_t* value);

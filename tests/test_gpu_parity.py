@@ -335,6 +335,45 @@ def test_fp16_split_sum_unit_plan(case):
     assert d <= 2 * MIXED_TOL
 
 
+LDS_HOP_CASES = [
+    # latt, Ne, num_nabla, links, (mode, ED_OPT_MIXED_WS): 16-byte (FP64), 8-byte complex64
+    # (FP32 split sum) and FP16 hi/lo pair workspaces; Lz = 3, the smallest z ring; Ne not a
+    # multiple of the eigenvector chunk; a lattice that does not tile (falls back)
+    ([16, 16, 16, 2], 40, 2, "haar", [("fp64", None), ("mixed", 1), ("mixed", 2)]),
+    ([8, 8, 3, 2], 24, 2, "haar", [("fp64", None)]),
+    ([16, 8, 32, 1], 37, 1, "smooth", [("fp64", None), ("mixed", 2)]),
+    ([6, 8, 8, 1], 20, 2, "haar", [("fp64", None)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(LDS_HOP_CASES)))
+def test_lds_staged_hop_is_bit_equal(case):
+    """The LDS-staged hop (ED_OPT_HOP_LDS 1: (x, y) tile with halo and a ring of
+    three z planes in LDS) writes the workspace the one-wave-per-site hop
+    (ED_OPT_HOP_LDS 0) writes, bit for bit, so every elemental is equal; so
+    does the default (-1), which picks one of them by element size."""
+    latt, Ne, nn, links, forms = LDS_HOP_CASES[case]
+    moms = S.momenta_by_norm(2)
+    U, V = S.make_inputs_np(latt, Ne, links, seed=120 + case)
+    for mode, ws in forms:
+        outs = {}
+        for lds in (0, 1, -1):
+            g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode=mode)
+            if ws is not None:
+                g.set_option(_native.ED_OPT_MIXED_STAGE2, 1)
+                g.set_option(_native.ED_OPT_MIXED_WS, ws)
+            g.set_option(_native.ED_OPT_HOP_LDS, lds)
+            g.load_arrays(U, V)
+            outs[lds] = g.calc_all()
+        same = all(np.array_equal(outs[0], outs[v]) for v in (1, -1))
+        say(f"LDS hop: latt={latt} Ne={Ne} nabla={nn} {mode} ws={ws}: bit-equal {same}")
+        assert same
+        if mode == "fp64" and np.prod(latt) <= 1024:
+            err = blockwise(outs[1], O.elemental(V, U, latt, nn, moms))
+            say(f"  fp64 vs oracle {err:.2e}")
+            assert err <= FP64_TOL
+
+
 def test_load_from_ildg_and_qdp_files(tmp_path):
     """load(key) through the reference's on-disk formats: ILDG/LIME links and
     QDP LazyDiskMapObj eigenvectors (elemental.py:102-105, preset.py:55-64,151-160)."""
