@@ -79,6 +79,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def mixed_precision(kernel):
+    """What runs in which precision in the mixed contraction kernels 2-5 (ED_Q_MIXED_KERNEL)."""
+    h16 = kernel == 5
+    return {
+        "stencil": "FP64 arithmetic, " + ("FP16 hi/lo pair storage of W 2^k (22 bits)" if h16 else "complex64 storage"),
+        "split_sum": "FP16 MFMA on the hi/lo pairs of both operands, FP32 sums" if h16 else "FP32 MFMA",
+        "phase_contraction": "FP16 MFMA on a three-term hi/lo split of phase and Q, FP32 partial sums per 32 sites",
+        "reduction_and_phase_table": "FP64",
+    }
+
+
 def blockwise_error(E, Eref, tables=False):
     """Error of E against Eref (torch tensors [Lt, D, P, Ne, Ne]) per (d, p) block.
 
@@ -356,8 +367,17 @@ def bench_batched(args):
     batch = ElementalBatch(gens)
     t0 = time.time()
     tuned_ms = batch.tune(1)
-    shape = {"grid_per_cu": batch.option(_native.ED_BOPT_GRID_PER_CU), "order": batch.option(_native.ED_BOPT_ORDER)}
+    shape = {
+        "grid_per_cu": batch.option(_native.ED_BOPT_GRID_PER_CU),
+        "order": batch.option(_native.ED_BOPT_ORDER),
+        "strip": batch.option(_native.ED_BOPT_STRIP),
+        "fp16_split_sum": batch.option(_native.ED_BOPT_H16),
+    }
     log(f"autotuned in {time.time() - t0:.1f} s: {tuned_ms:.1f} ms per run, {shape}")
+    if shape["fp16_split_sum"] == 1:
+        # the contexts' flop queries then describe the FP16 split-sum kernel the batch runs
+        for g in gens:
+            g.set_option(_native.ED_OPT_MIXED_WS, 2)
     for i in range(args.warmup):
         batch.run()
     torch.cuda.synchronize()
@@ -414,7 +434,8 @@ def bench_batched(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32+fp16x3",
+        "precision": mixed_precision(5 if shape["fp16_split_sum"] == 1 else 3),
         "data": "synthetic (seeded smooth SU(3) links eps=0.3, orthonormal complex64 eigenvectors)",
         "config": {
             "workload": " + ".join(f"{p['lattice']}/Nev{p['Nev']}" for p in per),
@@ -431,7 +452,9 @@ def bench_batched(args):
             "unit": "TFLOP/s",
             "frac": equiv / dt / 1e12 / PEAK["mixed"],
             "traffic": None,
-            "kernel": "whole graph (stencils + contract_mixed_group_kernel)",
+            "kernel": "whole graph (stencils + "
+            + ("contract_mixed16_item_kernel" if shape["fp16_split_sum"] == 1 else "contract_mixed_group_kernel")
+            + ")",
             "flops_basis": "executed FP32 MFMA flops plus FP16 MFMA flops at the 157.3:2500 peak ratio, over the whole step",
         },
         "per_lattice": per,
@@ -654,16 +677,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp16x3" if head == "mixed" and r["mixed_kernel"] in (2, 3, 4, 5) else (
             "fp32" if head == "mixed" else "fp64"),
-        "precision": (
-            {
-                "stencil": "FP64 arithmetic, complex64 storage",
-                "split_sum": "FP32 MFMA",
-                "phase_contraction": "FP16 MFMA on a three-term hi/lo split of phase and Q, FP32 partial sums per 32 sites",
-                "reduction_and_phase_table": "FP64",
-            }
-            if head == "mixed" and r["mixed_kernel"] in (2, 3, 4, 5)
-            else None
-        ),
+        "precision": mixed_precision(r["mixed_kernel"]) if head == "mixed" and r["mixed_kernel"] in (2, 3, 4, 5) else None,
         "data": "synthetic (seeded Haar SU(3) links, orthonormal complex64 eigenvectors)",
         "config": {
             "workload": f"{Lx}^3x{Lt}" if Lx == Ly == Lz else f"{Lx}x{Ly}x{Lz}x{Lt}",

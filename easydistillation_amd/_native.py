@@ -71,7 +71,7 @@ SIGNATURES = {
     "ed_batch_time": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double)]),
     "ed_batch_tune": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double)]),
 }
-ED_BOPT_GRID_PER_CU, ED_BOPT_ORDER, ED_BOPT_GRAPH, ED_BOPT_STRIP = 1, 2, 3, 4
+ED_BOPT_GRID_PER_CU, ED_BOPT_ORDER, ED_BOPT_GRAPH, ED_BOPT_STRIP, ED_BOPT_H16 = 1, 2, 3, 4, 5
 ED_Q_CONTRACT_FLOPS_PER_T, ED_Q_ALGO_FLOPS_PER_T, ED_Q_PRIMARY_DERIVATIVES = 1, 2, 3
 ED_Q_MOMENTUM_BLOCKS, ED_Q_HOPS_PER_T, ED_Q_SPLITS = 4, 5, 6
 ED_Q_MIXED_F32_FLOPS_PER_T, ED_Q_MIXED_F16_FLOPS_PER_T = 7, 8

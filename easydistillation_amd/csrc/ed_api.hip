@@ -750,9 +750,10 @@ int h16_unit_groups(const ed_context* c) { return (3 * c->nsplit_u_max + 7) / 8;
 // the unit plan runs when its k-groups fit the kernel
 bool h16_unit(const ed_context* c) { return c->h16_unit_opt && c->unit_ok && h16_unit_groups(c) <= 2; }
 int h16_groups(const ed_context* c) { return h16_unit(c) ? h16_unit_groups(c) : (3 * c->nsplit_max + 7) / 8; }
-bool use_h16(const ed_context* c) {
-  return use_f16_stage2(c) && ws_fits32(c) && c->mixed_ws == 2 && h16_groups(c) <= 2 && c->V3 >= 4096;
+bool h16_fits(const ed_context* c) {
+  return use_f16_stage2(c) && ws_fits32(c) && h16_groups(c) <= 2 && c->V3 >= 4096;
 }
+bool use_h16(const ed_context* c) { return c->mixed_ws == 2 && h16_fits(c); }
 
 // the contraction kernel the mixed form runs (the order of the tests in
 // run_batches): see ED_Q_MIXED_KERNEL
@@ -1222,12 +1223,15 @@ struct ed_batch {
   int grid_per_cu = 2;  // persistent grid = grid_per_cu x CUs; 0: one workgroup per item
   int order = 0;        // item order of the grouped kernels
   int strip = 0;        // tile-order strip height for every member (0: each context's own)
+  int h16 = -1;         // FP16 split sum for mixed members: -1 as each context's ED_OPT_MIXED_WS, 0 never, 1 where it fits
   int use_graph = 1;
   int ncu = 256;
-  // kinds: 0 FP64 MFMA, 1 FP32 MFMA (mixed, FP32 stage 2), 2 mixed FP16 stage 2
-  DevBuf<ed::GroupProblem> probs[3];
-  std::vector<ed::GroupProblem> host_probs[3];
-  int nprob[3] = {0, 0, 0}, total[3] = {0, 0, 0}, nks[3] = {1, 1, 1};
+  // kinds: 0 FP64 MFMA, 1 FP32 MFMA (mixed, FP32 stage 2), 2 mixed FP16 stage 2,
+  // 3 mixed with the FP16 split sum as well (hi/lo FP16 workspace)
+  static constexpr int NKIND = 4;
+  DevBuf<ed::GroupProblem> probs[NKIND];
+  std::vector<ed::GroupProblem> host_probs[NKIND];
+  int nprob[NKIND] = {0, 0, 0, 0}, total[NKIND] = {0, 0, 0, 0}, nks[NKIND] = {1, 1, 1, 1};
   // kind 2 runs the 64-bit-address form (some lattice's timeslice workspace
   // reaches 2^31 bytes, or ED_OPT_WIDE_WS is set on one)
   bool wide = false;
@@ -1276,15 +1280,17 @@ int bguard(ed_batch* b, const std::function<void()>& body) {
   }
 }
 
-int batch_kind(const ed_context* c, int mode) {
+int batch_kind(const ed_batch* b, const ed_context* c, int mode) {
   if (mode == ED_MODE_FP64 || mixed_runs_fp64(c)) return 0;
-  return use_f16_stage2(c) ? 2 : 1;
+  if (!use_f16_stage2(c)) return 1;
+  const bool h16 = b->h16 < 0 ? use_h16(c) : (b->h16 > 0 && h16_fits(c));
+  return h16 ? 3 : 2;
 }
 
 // Plans, workspaces (every timeslice at once) and the grouped problem tables;
 // returns the fingerprint of everything a captured graph refers to.
 std::vector<uint64_t> batch_prepare(ed_batch* b) {
-  std::vector<ed::GroupProblem> tab[3];
+  std::vector<ed::GroupProblem> tab[ed_batch::NKIND];
   std::vector<uint64_t> fp;
   b->wide = false;
   for (auto& it : b->items) {
@@ -1293,11 +1299,21 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
             "every context of a batch needs lattice, gauge, eigenvectors and momenta");
     HIPCHK(hipSetDevice(c->device));
     build_plan(c);
-    const int kind = batch_kind(c, it.mode);
+    const int kind = batch_kind(b, c, it.mode);
     void* W = kind == 0 ? (void*)ensure_workspace<double>(c, c->Lt) : (void*)ensure_workspace<float>(c, c->Lt);
     ed::GroupProblem gp{};
     gp.a = contract_args(c, c->Lt, 0);
     if (b->strip > 0) gp.a.strip = b->strip;
+    if (kind == 3) {
+      // the same bytes hold the hi/lo FP16 pairs (as in run_batches)
+      c->wsh = h16_shift(c);
+      gp.a.g.wsh = c->wsh;
+      if (h16_unit(c)) {
+        gp.a.unit = 1;
+        gp.a.dplan = c->dplan_u.p;
+        gp.a.splits = c->splits_u.p;
+      }
+    }
     gp.W = W;
     if (kind == 0) {
       const size_t phn = (size_t)c->npb * 16 * c->V3P;
@@ -1314,7 +1330,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
     gp.out = it.out;
     gp.work0 = b->total[kind];
     tab[kind].push_back(gp);
-    b->nks[kind] = std::max(b->nks[kind], c->nks_max);
+    b->nks[kind] = std::max(b->nks[kind], kind == 3 ? h16_groups(c) : c->nks_max);
     if (kind == 2 && ws_wide(c)) b->wide = true;
     fp.push_back((uint64_t)(uintptr_t)c);
     fp.push_back(c->gen);
@@ -1322,6 +1338,8 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
     fp.push_back((uint64_t)(uintptr_t)c->U64.p);
     fp.push_back((uint64_t)(uintptr_t)c->V.p);
     fp.push_back((uint64_t)(uintptr_t)c->V64.p);
+    fp.push_back((uint64_t)kind);
+    fp.push_back((uint64_t)(int64_t)c->wsh);
     int64_t tot = (int64_t)b->total[kind] + gp.a.nwork;
     if (tot >= ((int64_t)1 << 31)) fail(ED_ERR_UNSUPPORTED, "batch has too many work items");
     b->total[kind] = (int)tot;
@@ -1330,7 +1348,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
   require(!(b->wide && b->nks[2] > 3), ED_ERR_UNSUPPORTED,
           "a lattice with 2^31 or more bytes of workspace per timeslice shares the mixed batch with "
           "derivatives of more than four split terms; batch them separately");
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < ed_batch::NKIND; ++k) {
     b->nprob[k] = (int)tab[k].size();
     if (tab[k].empty()) continue;
     const size_t bytes = tab[k].size() * sizeof(ed::GroupProblem);
@@ -1349,6 +1367,7 @@ std::vector<uint64_t> batch_prepare(ed_batch* b) {
   fp.push_back((uint64_t)b->order);
   fp.push_back((uint64_t)b->strip);
   fp.push_back((uint64_t)b->wide);
+  fp.push_back((uint64_t)(int64_t)b->h16);
   return fp;
 }
 
@@ -1366,12 +1385,27 @@ void batch_enqueue(ed_batch* b) {
   };
   for (auto& it : b->items) {
     ed_context* c = it.ctx;
-    if (batch_kind(c, it.mode) == 0)
+    const int kind = batch_kind(b, c, it.mode);
+    if (kind == 0)
       enqueue_stencil<double>(c, (double2*)(void*)c->W.p, 0, c->Lt, b->stream, direct);
+    else if (kind == 3)
+      enqueue_stencil<ed::hsplit>(c, (uint2*)(void*)c->W.p, 0, c->Lt, b->stream, direct);
     else
       enqueue_stencil<float>(c, (float2*)(void*)c->W.p, 0, c->Lt, b->stream, direct);
   }
   const dim3 blk(64 * ed::NWAVE);
+  if (b->nprob[3]) {
+    // FP16 split sum: one workgroup per work item (the persistent shapes are
+    // not built for it), at most two k-groups (h16_fits)
+    const int tot = b->total[3];
+    if (b->nks[3] <= 1)
+      ed::contract_mixed16_item_kernel<1><<<(tot + 7) / 8 * 8, blk, 0, b->stream>>>(b->probs[3].p, b->nprob[3], tot,
+                                                                                    b->order);
+    else
+      ed::contract_mixed16_item_kernel<2><<<(tot + 7) / 8 * 8, blk, 0, b->stream>>>(b->probs[3].p, b->nprob[3], tot,
+                                                                                    b->order);
+    HIPCHK(hipGetLastError());
+  }
   for (int k = 0; k < 3; ++k) {
     if (!b->nprob[k]) continue;
     const int grid = batch_grid(b, b->total[k]);
@@ -1428,7 +1462,7 @@ void batch_release_members(ed_batch* b) {
 void batch_run(ed_batch* b) {
   require(!b->items.empty(), ED_ERR_STATE, "the batch is empty");
   HIPCHK(hipSetDevice(b->device));
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < ed_batch::NKIND; ++k) {
     b->total[k] = 0;
     b->nks[k] = 1;
   }
@@ -2199,6 +2233,10 @@ int ed_batch_set_option(ed_batch* b, int option, int64_t value) {
         if (value < 0 || value > 64) fail(ED_ERR_ARG, "strip height must be in [0, 64]");
         b->strip = (int)value;
         break;
+      case ED_BOPT_H16:
+        if (value < -1 || value > 1) fail(ED_ERR_ARG, "ED_BOPT_H16 must be -1, 0 or 1");
+        b->h16 = (int)value;
+        break;
       default:
         fail(ED_ERR_ARG, "unknown batch option %d", option);
     }
@@ -2215,6 +2253,7 @@ int ed_batch_get_option(ed_batch* b, int option, int64_t* value) {
       case ED_BOPT_ORDER: *value = b->order; break;
       case ED_BOPT_GRAPH: *value = b->use_graph; break;
       case ED_BOPT_STRIP: *value = b->strip; break;
+      case ED_BOPT_H16: *value = b->h16; break;
       default: fail(ED_ERR_ARG, "unknown batch option %d", option);
     }
   });
@@ -2248,25 +2287,40 @@ int ed_batch_tune(ed_batch* b, int reps, double* best_ms) {
     const int grids[] = {1, 2, 4, 0};
     const int strips[] = {1, 2, 4, 8};
     double best = 1e300;
-    int bg = b->grid_per_cu, bo = b->order, bs = b->strip;
-    for (int gsel : grids)
-      for (int o = 0; o < 2; ++o)
-        for (int st : strips) {
-          b->grid_per_cu = gsel;
-          b->order = o;
-          b->strip = st;
-          b->drop_graph();
-          const double ms = batch_time(b, reps);
-          if (ms < best) {
-            best = ms;
-            bg = gsel;
-            bo = o;
-            bs = st;
+    int bg = b->grid_per_cu, bo = b->order, bs = b->strip, bh = b->h16;
+    // FP16 split sum off, then on where it fits (only when some member can
+    // take it; its kernel has the one-workgroup-per-item shape only)
+    bool any_h16 = false;
+    for (auto& it : b->items) {
+      HIPCHK(hipSetDevice(it.ctx->device));
+      build_plan(it.ctx);
+      any_h16 = any_h16 || (it.mode != ED_MODE_FP64 && !mixed_runs_fp64(it.ctx) && h16_fits(it.ctx));
+    }
+    HIPCHK(hipSetDevice(b->device));
+    for (int h : {0, 1}) {
+      if (h && !any_h16) continue;
+      for (int gsel : grids)
+        for (int o = 0; o < 2; ++o)
+          for (int st : strips) {
+            b->h16 = h;
+            b->grid_per_cu = gsel;
+            b->order = o;
+            b->strip = st;
+            b->drop_graph();
+            const double ms = batch_time(b, reps);
+            if (ms < best) {
+              best = ms;
+              bg = gsel;
+              bo = o;
+              bs = st;
+              bh = h;
+            }
           }
-        }
+    }
     b->grid_per_cu = bg;
     b->order = bo;
     b->strip = bs;
+    b->h16 = bh;
     b->drop_graph();
     if (best_ms) *best_ms = best;
   });

@@ -440,4 +440,21 @@ __global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed16_kernel(const u
   contract_mixed16_body<NKS>(W, ph16, out, a, w, Qf, wmax, PHs);
 }
 
+// Batched form: one workgroup per work item of the problem table (items
+// numbered problem after problem, mapped to XCDs as contract_mixed_item_kernel
+// does); W of each problem holds the hi/lo FP16 workspace.
+template <int NKS>
+__global__ __launch_bounds__(64 * NWAVE, 1) void contract_mixed16_item_kernel(const GroupProblem* __restrict__ probs,
+                                                                              int nprob, int total, int order) {
+  __shared__ __align__(16) float Qf[QFLOATS];
+  __shared__ float wmax[2][NWAVE];
+  __shared__ half8 PHs[2][4][64];
+  const int item = __builtin_amdgcn_readfirstlane(group_item(blockIdx.x >> 3, (total + 7) >> 3, order));
+  if (item >= total) return;
+  const int i = __builtin_amdgcn_readfirstlane(group_find(probs, nprob, item));
+  const ContractArgs a = probs[i].a;
+  contract_mixed16_body<NKS>((const uint2*)probs[i].W, (const half8*)probs[i].ph0, probs[i].out, a,
+                             item - probs[i].work0, Qf, wmax, PHs);
+}
+
 }  // namespace ed

@@ -343,14 +343,19 @@ int ed_get_gauge(ed_context* ctx, void* U, int on_device);
  *   ed_batch_tune: autotuner -- times every launch shape (persistent grid of
  *                 1, 2 or 4 workgroups per CU, or one workgroup per work item)
  *                 x item order (contiguous ranges per XCD, or round robin
- *                 over XCDs) x tile-order strip height (1, 2, 4, 8) with reps
- *                 runs each, keeps the fastest and returns its time. */
+ *                 over XCDs) x tile-order strip height (1, 2, 4, 8) x
+ *                 ED_BOPT_H16 (0, and 1 when some mixed member fits it) with
+ *                 reps runs each, keeps the fastest and returns its time. */
 typedef struct ed_batch ed_batch;
 enum ed_batch_option {
   ED_BOPT_GRID_PER_CU = 1, /* persistent workgroups per CU (0: one per work item; default 2) */
   ED_BOPT_ORDER = 2,       /* 0: contiguous item ranges per XCD (default), 1: round robin     */
   ED_BOPT_GRAPH = 3,       /* 1: capture and replay a hipGraph (default), 0: direct launches  */
-  ED_BOPT_STRIP = 4        /* tile-order strip height for every member (0: each context's own) */
+  ED_BOPT_STRIP = 4,       /* tile-order strip height for every member (0: each context's own) */
+  ED_BOPT_H16 = 5          /* mixed members' split sum on FP16 MFMA (the hi/lo FP16 workspace of
+                              ED_OPT_MIXED_WS 2, one workgroup per work item): -1 (default) as each
+                              context's ED_OPT_MIXED_WS, 1 wherever it fits (V3 >= 4096, at most two
+                              k-groups, < 2^31 workspace bytes per timeslice), 0 never */
 };
 int ed_batch_create(int device, ed_batch** b);
 int ed_batch_destroy(ed_batch* b);

@@ -115,15 +115,65 @@ def test_recapture_after_input_change():
 
 def test_tune_picks_a_valid_shape():
     gens, _ = make_generators(SPECS, seed=40)
-    singles = [g.calc_all() for g in gens]
     b = ElementalBatch(gens)
     ms = b.tune(1)
     grid, order = b.option(_native.ED_BOPT_GRID_PER_CU), b.option(_native.ED_BOPT_ORDER)
-    say(f"tuned: {ms:.3f} ms per run, grid/CU={grid} order={order}")
-    assert ms > 0 and grid in (0, 1, 2, 4) and order in (0, 1)
+    h16 = b.option(_native.ED_BOPT_H16)
+    say(f"tuned: {ms:.3f} ms per run, grid/CU={grid} order={order} h16={h16}")
+    assert ms > 0 and grid in (0, 1, 2, 4) and order in (0, 1) and h16 in (0, 1)
     outs = [o.cpu().numpy() for o in b.run()]
-    for s, o in zip(singles, outs):
-        assert np.abs(o - s).max() == 0
+    # the single-context runs take the split-sum kernel the tuner chose
+    for g in gens:
+        g.set_option(_native.ED_OPT_MIXED_WS, 2 if h16 == 1 else 1)
+    for g, o in zip(gens, outs):
+        assert np.abs(o - g.calc_all()).max() == 0
+
+
+H16_SPECS = [
+    # the FP16 split sum fits the 16^3 mixed members (V3 >= 4096); the others keep their kinds
+    ([16, 16, 16, 1], 40, 2, S.momenta_by_norm(2), "mixed", "haar"),
+    ([6, 5, 4, 2], 24, 2, S.unit_momenta(), "mixed", "haar"),
+    ([16, 16, 16, 3], 24, 1, S.momenta_by_norm(1), "mixed", "smooth"),
+    ([8, 8, 8, 2], 20, 1, S.momenta_by_norm(2), "fp64", "haar"),
+]
+
+
+@pytest.mark.parametrize("grid,order", [(0, 1), (2, 0)])
+def test_batch_fp16_split_sum(grid, order):
+    """ED_BOPT_H16 1 runs the FP16 split sum (hi/lo FP16 workspace,
+    contract_mixed16 arithmetic) for the mixed members it fits: their outputs
+    equal the single-context ED_OPT_MIXED_WS 2 runs bit for bit (kernel 5 by
+    ED_Q_MIXED_KERNEL), differ from the FP32 split sum of ED_BOPT_H16 0, and
+    meet the mixed bar against the oracle; ED_BOPT_H16 -1 follows each
+    context's ED_OPT_MIXED_WS."""
+    gens, inputs = make_generators(H16_SPECS, seed=70)
+    b = ElementalBatch(gens)
+    b.set_option(_native.ED_BOPT_GRID_PER_CU, grid)
+    b.set_option(_native.ED_BOPT_ORDER, order)
+    b.set_option(_native.ED_BOPT_H16, 0)
+    off = [o.cpu().numpy() for o in b.run()]
+    b.set_option(_native.ED_BOPT_H16, 1)
+    on = [o.cpu().numpy() for o in b.run()]
+    for (latt, Ne, nn, moms, mode, _), g, o0, o1, (U, V) in zip(H16_SPECS, gens, off, on, inputs):
+        fits = mode == "mixed" and np.prod(latt[:3]) >= 4096
+        g.set_option(_native.ED_OPT_MIXED_WS, 2)
+        single = g.calc_all()
+        kernel = g.context.query(_native.ED_Q_MIXED_KERNEL) if mode == "mixed" else None
+        err = blockwise(o1, O.elemental(V, U, latt, nn, moms))
+        say(f"h16 batch {latt} Ne={Ne} nabla={nn} {mode}: kernel {kernel}, |batch - single| "
+            f"{np.abs(o1 - single).max():.1e}, |h16 - fp32 split| {np.abs(o1 - o0).max():.1e}, vs oracle {err:.2e}")
+        assert np.array_equal(o1, single)
+        assert err <= (1e-12 if mode == "fp64" else 1e-6)
+        if fits:
+            assert kernel == 5
+            assert not np.array_equal(o1, o0)
+        else:
+            assert np.array_equal(o1, o0)
+    # -1: as each context's ED_OPT_MIXED_WS (now 2 on every member)
+    b.set_option(_native.ED_BOPT_H16, -1)
+    follow = [o.cpu().numpy() for o in b.run()]
+    for o1, f in zip(on, follow):
+        assert np.array_equal(o1, f)
 
 
 def test_batch_rejects_duplicates_and_empty():

@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 
 import easydistillation_amd as ed
+from easydistillation_amd import _native
 from easydistillation_amd import synthetic as S
 from easydistillation_amd.batch import ElementalBatch
 from oracle import elemental_oracle as O
@@ -71,8 +72,12 @@ def test_config_d_workload():
 
     batch = ElementalBatch(gens)
     ms = batch.tune(1)
-    say(f"config D: tuned batch {ms:.1f} ms per replay")
+    h16 = batch.option(_native.ED_BOPT_H16)
+    say(f"config D: tuned batch {ms:.1f} ms per replay, FP16 split sum {h16}")
     outs = batch.run()
+    # the single-context runs below take the split-sum kernel the tuner chose
+    for g in gens:
+        g.set_option(_native.ED_OPT_MIXED_WS, 2 if h16 == 1 else 1)
 
     for (latt, Ne), g, out, (U, V) in zip(LATTICES, gens, outs, inputs):
         Lt = latt[3]
