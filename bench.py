@@ -637,6 +637,10 @@ def main():
         nt = min(Lt, args.ref_loop_t)
         g = ed.ElementalGenerator(latt, None, None, num_nabla, moms, Ne=Ne, mode=head)
         g.load_arrays(U, V)
+        if not args.no_tune:
+            from easydistillation_amd.tune import autotune
+
+            autotune(g, cache=args.tune_cache)  # the shape's table entry, as for the timed forms
         data = np.zeros((nt, D, P, Ne, Ne), dtype=np.complex128)
         g.calc(0)  # warm-up: kernels, page-locked staging buffer
         g.load_arrays(U, V)  # drops the computed window
