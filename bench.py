@@ -658,12 +658,36 @@ def main():
             "identity_check_max_abs": chk,
             "note": (
                 f"'for t: data[t] = calc(t)' into a preallocated numpy array; calc(t) computes aligned "
-                f"windows of {g.WINDOW} timeslices per launch into HBM and copies each timeslice out through "
-                f"a page-locked buffer ({D * P * Ne * Ne * 16 / 1e6:.0f} MB per timeslice, PCIe included)"
+                f"windows of {g.WINDOW} timeslices per launch into HBM (the next window computing while the "
+                f"host takes this one) and copies each timeslice out through page-locked buffers (the next "
+                f"timeslice's copy overlapping the host copy; {D * P * Ne * Ne * 16 / 1e6:.0f} MB per "
+                f"timeslice, PCIe included)"
             ),
         }
-        log(f"reference-style loop: {json.dumps(ref_loop)}")
-        del g, data
+        # the same loop with the array in device memory, as the reference's driver
+        # runs it on its cupy backend (backend.zeros on the GPU)
+        from easydistillation_amd.backend import get_backend, set_backend
+
+        before = get_backend()
+        set_backend("torch")
+        try:
+            g.load_arrays(U, V)  # drops the computed window
+            ddev = torch.zeros((nt, D, P, Ne, Ne), dtype=torch.complex128, device=dev)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for t in range(nt):
+                ddev[t] = g.calc(t)
+            torch.cuda.synchronize()
+            dt_dev = (time.perf_counter() - t1) / nt
+        finally:
+            set_backend(before)
+        ref_loop["device_array"] = {
+            "ms_per_timeslice": dt_dev * 1e3,
+            "elementals_per_s": P * D * Ne * Ne / dt_dev,
+            "note": "'for t: data[t] = calc(t)' with data a device tensor (the reference driver on its cupy backend)",
+        }
+        log(f"reference-style loop, device array: {json.dumps(ref_loop['device_array'])}")
+        del g, data, ddev
     # the measured bytes hold for the library's own timeslice batch (8 per launch at this workload)
     strip_eff = r["strip"] or (4 if head == "mixed" and r["mixed_kernel"] else 2)
     all_f16 = head == "mixed" and r["mixed_kernel"] == 5

@@ -135,7 +135,10 @@ class ElementalGenerator:
         self._loaded = False
         self._all = None  # (mode, [Lt, D, P, Ne, Ne]) of the last calc_all
         self._win = None  # (mode, t0, device tensor [n, D, P, Ne, Ne]) of the last calc(t) window
-        self._pinned = None  # page-locked host staging buffer of one timeslice
+        self._next = None  # the following window, its compute enqueued and not waited for
+        self._pins = None  # two page-locked host buffers of one timeslice
+        self._pre = None  # ((mode, t), pin index, event) of the copy to page-locked memory in flight
+        self._copy_stream = None
 
     # ------------------------------------------------------------------ input
     def load(self, key: str):
@@ -175,6 +178,7 @@ class ElementalGenerator:
             raise ValueError(f"U shape {tuple(U.shape)} is not [3, Lt, Lz, Ly, Lx, 3, 3]")
         if tuple(V.shape) != (Lt, self.Ne, Lz, Ly, Lx, Nc):
             raise ValueError(f"V shape {tuple(V.shape)} is not [Lt, Ne, Lz, Ly, Lx, 3]")
+        self._drain()
         ptr, dev, code, keep = _device_ptr(U)
         self._ctx.set_gauge(ptr, code, dev)
         del keep
@@ -228,7 +232,19 @@ class ElementalGenerator:
 
     WINDOW = 8  # timeslices per device window of calc(t)
 
+    def _drain(self) -> None:
+        """Waits for what calc(t) left in flight (the next window's compute on the
+        library stream, the next timeslice's copy to page-locked memory) and
+        drops it, so that no buffer is released or overwritten while it is used."""
+        if self._pre is not None:
+            self._pre[2].synchronize()
+            self._pre = None
+        if self._next is not None:
+            self._ctx.synchronize()
+            self._next = None
+
     def _invalidate(self) -> None:
+        self._drain()
         self._all = None
         self._win = None
 
@@ -252,26 +268,73 @@ class ElementalGenerator:
             return res.clone() if get_backend() == "torch" else np.array(res, copy=True)
         import torch
 
+        host = get_backend() != "torch"
+        dev = f"cuda:{self._ctx.device}"
+
+        def window_shape(t0):
+            return (min(self.WINDOW, Lt - t0), self.num_derivative, self.num_momentum, self.Ne, self.Ne)
+
         w = self._win
         if w is None or w[0] != m or not w[1] <= t < w[1] + w[2].shape[0]:
+            nx = self._next
             t0 = t // self.WINDOW * self.WINDOW
-            n = min(self.WINDOW, Lt - t0)
-            shape = (n, self.num_derivative, self.num_momentum, self.Ne, self.Ne)
-            buf = w[2] if w is not None and tuple(w[2].shape) == shape else None
-            self._win = None
-            if buf is None:
-                buf = torch.empty(shape, dtype=torch.complex128, device=f"cuda:{self._ctx.device}")
-            self._ctx.compute(t0, n, self._mode_code(m), buf.data_ptr(), True)
-            self._ctx.synchronize()
-            self._win = w = (m, t0, buf)
+            if nx is not None and nx[0] == m and nx[1] == t0:
+                # the window enqueued by the previous calls
+                if self._pre is not None:
+                    self._pre[2].synchronize()  # the last copy out of the old window
+                    self._pre = None
+                self._ctx.synchronize()
+                self._next = None
+                spare = w[2] if w is not None else None
+                self._win = w = nx
+            else:
+                self._drain()
+                spare = nx[2] if nx is not None else None
+                buf = w[2] if w is not None and tuple(w[2].shape) == window_shape(t0) else None
+                self._win = None
+                if buf is None:
+                    buf = torch.empty(window_shape(t0), dtype=torch.complex128, device=dev)
+                torch.cuda.current_stream(dev).synchronize()  # buf's memory is idle on torch's stream
+                self._ctx.compute(t0, buf.shape[0], self._mode_code(m), buf.data_ptr(), True)
+                self._ctx.synchronize()
+                spare = spare if spare is not None else (w[2] if w is not None and w[2] is not buf else None)
+                self._win = w = (m, t0, buf)
+            t1 = w[1] + self.WINDOW
+            if host and t1 < Lt:
+                # the following window computes while this one is copied to the host
+                shape = window_shape(t1)
+                nb = spare if spare is not None and tuple(spare.shape) == shape else None
+                if nb is None:
+                    nb = torch.empty(shape, dtype=torch.complex128, device=dev)
+                    torch.cuda.current_stream(dev).synchronize()
+                self._ctx.compute(t1, shape[0], self._mode_code(m), nb.data_ptr(), True)
+                self._next = (m, t1, nb)
         res = w[2][t - w[1]]
-        if get_backend() == "torch":
+        if not host:
             return res.clone()
-        if self._pinned is None or tuple(self._pinned.shape) != tuple(res.shape):
-            self._pinned = torch.empty(tuple(res.shape), dtype=torch.complex128, pin_memory=True)
-        self._pinned.copy_(res)
+        # timeslice t in page-locked memory: copied ahead by the previous call, or now
+        if self._pins is None or tuple(self._pins[0].shape) != tuple(res.shape):
+            self._pins = [torch.empty(tuple(res.shape), dtype=torch.complex128, pin_memory=True) for _ in range(2)]
+            self._copy_stream = torch.cuda.Stream(device=dev)
+        pre = self._pre
+        self._pre = None
+        if pre is not None:
+            pre[2].synchronize()
+        if pre is not None and pre[0] == (m, t):
+            k = pre[1]
+        else:
+            k = 0
+            self._pins[k].copy_(res)
+        # start the copy of t + 1 into the other buffer, then copy t to fresh host memory
+        if t + 1 < w[1] + w[2].shape[0]:
+            src = w[2][t + 1 - w[1]]
+            with torch.cuda.stream(self._copy_stream):
+                self._pins[1 - k].copy_(src, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(self._copy_stream)
+            self._pre = ((m, t + 1), 1 - k, ev)
         # multi-threaded host copy into fresh memory, returned as a numpy array
-        return torch.empty(tuple(res.shape), dtype=torch.complex128).copy_(self._pinned).numpy()
+        return torch.empty(tuple(res.shape), dtype=torch.complex128).copy_(self._pins[k]).numpy()
 
     def calc_range(self, t0: int, nt: int, mode: str | None = None, out=None):
         """[nt, D, P, Ne, Ne] for timeslices t0..t0+nt-1 in batched launches.
@@ -311,6 +374,7 @@ class ElementalGenerator:
         return res
 
     def set_option(self, option: int, value: int) -> None:
+        self._drain()
         self._ctx.set_option(option, value)
         self._invalidate()
 

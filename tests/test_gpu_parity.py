@@ -175,6 +175,45 @@ def test_calc_windows_any_order_and_reload():
         assert blockwise(g.calc(t)[None], ref2[t][None]) <= FP64_TOL, t
 
 
+def test_calc_loop_overlaps_and_matches_calc_range():
+    """The reference driver's loop, for t: data[t] = calc(t), with the next
+    window computing and the next timeslice copying to page-locked memory
+    while the host takes the current one: every timeslice equals calc_range's
+    bit for bit, also across an option change in the middle of a window (the
+    work in flight is waited for) and with the "torch" backend."""
+    import torch
+
+    from easydistillation_amd.backend import get_backend, set_backend
+
+    latt, Ne, nn = [8, 8, 8, 11], 16, 2
+    moms = S.momenta_by_norm(2)
+    U, V = S.make_inputs_np(latt, Ne, "haar", seed=17)
+    Lt = latt[3]
+    for mode in ("fp64", "mixed"):
+        g = ed.ElementalGenerator(latt, None, None, nn, moms, Ne=Ne, mode=mode)
+        g.WINDOW = 4  # windows [0, 4), [4, 8), [8, 11)
+        g.load_arrays(U, V)
+        ref = g.calc_range(0, Lt)
+        data = np.zeros_like(ref)
+        for t in range(Lt):
+            data[t] = g.calc(t)
+            if t == 5:
+                g.set_option(_native.ED_OPT_HERMITIAN, 1)  # the default again: same results
+        say(f"calc loop {mode}: equal to calc_range {np.array_equal(data, ref)}")
+        assert np.array_equal(data, ref)
+        for t in [9, 10, 2, 3, 4]:
+            assert np.array_equal(g.calc(t), ref[t]), t
+        before = get_backend()
+        set_backend("torch")
+        try:
+            dd = torch.zeros(ref.shape, dtype=torch.complex128, device="cuda")
+            for t in range(Lt):
+                dd[t] = g.calc(t)
+            assert np.array_equal(dd.cpu().numpy(), ref)
+        finally:
+            set_backend(before)
+
+
 def test_load_from_containers(tmp_path):
     latt = [4, 4, 4, 3]
     Lx, Ly, Lz, Lt = latt
