@@ -382,6 +382,9 @@ LDS_HOP_CASES = [
     ([8, 8, 3, 2], 24, 2, "haar", [("fp64", None)]),
     ([16, 8, 32, 1], 37, 1, "smooth", [("fp64", None), ("mixed", 2)]),
     ([6, 8, 8, 1], 20, 2, "haar", [("fp64", None)]),
+    # three derivatives (level 3: nine parents) and a single eigenvector
+    ([8, 8, 4, 1], 9, 3, "haar", [("fp64", None), ("mixed", 2)]),
+    ([8, 8, 3, 1], 1, 2, "smooth", [("fp64", None)]),
 ]
 
 
