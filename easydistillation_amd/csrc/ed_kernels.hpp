@@ -268,11 +268,11 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
 
 // --------------------------------------------------------------------------
 // LDS-staged hop (same outputs as hop3_kernel, bit for bit).  One workgroup:
-// one parent slot, HL_EC(T) eigenvectors (16 for 8-byte workspace elements,
+// one parent slot, hl_ec<T>() eigenvectors (16 for 8-byte workspace elements,
 // 8 for 16-byte ones: 128 bytes per site and colour), one 8 x 8 tile of
 // (x, y) sites, marching over all z.  LDS holds
 //   * a ring of three planes (z - 1, z, z + 1) of the parent: 10 x 10 sites
-//     (the tile and a one-site halo) x 3 colours x HL_EC eigenvectors, and
+//     (the tile and a one-site halo) x 3 colours x hl_ec<T>() eigenvectors, and
 //   * the links the plane's 64 outputs need: U_x and U_y on the tile, U_x on
 //     the halo column x0 - 1 and U_y on the halo row y0 - 1 (backward links),
 //     and U_z on the tile for the planes z and z - 1 (two slots),
@@ -284,8 +284,8 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
 // colours of its site's child elements from one read of the six neighbours
 // (the arithmetic of hop3_kernel, in its order); wave w of NW takes the
 // plane's sites SPI w ... SPI (w + 1) - 1, then SPI (w + NW) ...
-// Requires Lx, Ly multiples of 8, Lz >= 3, NeP a multiple of HL_EC(T).
-// grid (Lx/8 * Ly/8, nparent * NeP/HL_EC, TB), block 64 * NW
+// Requires Lx, Ly multiples of 8, Lz >= 3, NeP a multiple of hl_ec<T>().
+// grid (Lx/8 * Ly/8, nparent * NeP/hl_ec<T>(), TB), block 64 * NW
 // --------------------------------------------------------------------------
 constexpr int HL_T = 8, HL_H = HL_T + 2, HL_SITES = HL_H * HL_H;
 constexpr int HL_PSI_PIECES = HL_SITES * 3 * 8;      // 16-byte pieces of one plane (2400)
