@@ -302,13 +302,18 @@ class ElementalGenerator:
             t1 = w[1] + self.WINDOW
             if host and t1 < Lt:
                 # the following window computes while this one is copied to the host
+                # (skipped when device memory cannot hold a second window)
                 shape = window_shape(t1)
                 nb = spare if spare is not None and tuple(spare.shape) == shape else None
                 if nb is None:
-                    nb = torch.empty(shape, dtype=torch.complex128, device=dev)
+                    try:
+                        nb = torch.empty(shape, dtype=torch.complex128, device=dev)
+                    except torch.cuda.OutOfMemoryError:
+                        nb = None
                     torch.cuda.current_stream(dev).synchronize()
-                self._ctx.compute(t1, shape[0], self._mode_code(m), nb.data_ptr(), True)
-                self._next = (m, t1, nb)
+                if nb is not None:
+                    self._ctx.compute(t1, shape[0], self._mode_code(m), nb.data_ptr(), True)
+                    self._next = (m, t1, nb)
         res = w[2][t - w[1]]
         if not host:
             return res.clone()
