@@ -262,6 +262,7 @@ def bench_differentiation(args):
     w = torch.rand(Ne, dtype=torch.float64, device=dev, generator=gen) + 0.5
     log(f"inputs in {time.time() - t0:.1f} s")
     eng = ElementalEngine(latt, Ne, num_nabla, moms)
+    eng.ctx.set_option(_native.ED_OPT_HOP_LDS, args.hop_lds)
     Ug, Vg, wg = (x.clone().requires_grad_() for x in (U, V, w))
     del U, V
 
