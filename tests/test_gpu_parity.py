@@ -283,6 +283,13 @@ def test_config_b_shape_properties():
     err = blockwise(Em, E)
     say(f"  mixed vs fp64 err={err:.3e}")
     assert err <= MIXED_TOL
+    # the tuner's choice at this shape: the FP16 split sum (unit plan)
+    g.set_option(_native.ED_OPT_MIXED_WS, 2)
+    Eh = g.calc_all(mode="mixed")
+    assert g.context.query(_native.ED_Q_MIXED_KERNEL) == 5
+    err16 = blockwise(Eh, E)
+    say(f"  FP16 split sum vs fp64 err={err16:.3e}")
+    assert err16 <= MIXED_TOL
     # one timeslice against the oracle
     say("config B shape: oracle t=0")
     Uh, Vh = U.cpu().numpy(), V.cpu().numpy()
