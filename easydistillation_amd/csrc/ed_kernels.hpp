@@ -968,25 +968,28 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // Three-term split of two FP32 values: hi = fp16(x) (round to nearest even,
-// packed), lo = fp16(x - hi) with the residual formed exactly inside one
-// v_fma_mix per value (fma(hi, m1, x) with m1 = -1 a runtime value, so the
-// compiler keeps the fused form instead of folding it to a subtraction).
-// Plain code, not inline assembly: the results feed matrix instructions,
-// and the compiler inserts the wait states a vector write before a matrix
-// read needs only for instructions it sees.
-// z0 is a runtime zero added to the packed hi: the compiler then reads both
-// FP16 halves of that one register in the two v_fma_mix (operand selection)
-// instead of converting x0 and x1 to FP16 a second time.
-__device__ __forceinline__ void split16x2(float x0, float x1, float m1, uint32_t z0, uint32_t& hi, uint32_t& lo) {
+// one packed v_cvt_pk_f16_f32), lo = fp16(x - hi) with the residual formed
+// exactly inside one v_fma_mix per value (fma(hi, m1, x) with m1 = -1 a
+// runtime value, so the compiler keeps the fused form instead of folding it
+// to a subtraction), reading hi's two FP16 halves by operand selection:
+// 3 instructions per pair.  The two asm statements are empty (they emit no
+// instruction): they make x and hi opaque, so the compiler neither fuses the
+// conversion with the FMA that produced x (computing x a second time) nor
+// converts x0 and x1 to FP16 again for the residual.  Every instruction is
+// the compiler's own, so it inserts the wait states a vector write before a
+// matrix read needs.
+__device__ __forceinline__ void split16x2(float x0, float x1, float m1, uint32_t& hi, uint32_t& lo) {
   typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-  hi = __builtin_bit_cast(uint32_t, h2{(_Float16)x0, (_Float16)x1}) + z0;
-  const h2 h = __builtin_bit_cast(h2, hi);
+  asm volatile("" : "+v"(x0), "+v"(x1));
+  uint32_t hb = __builtin_bit_cast(uint32_t, h2{(_Float16)x0, (_Float16)x1});
+  asm volatile("" : "+v"(hb));
+  hi = hb;
+  const h2 h = __builtin_bit_cast(h2, hb);
   lo = __builtin_bit_cast(uint32_t, h2{(_Float16)__builtin_fmaf((float)h.x, m1, x0), (_Float16)__builtin_fmaf((float)h.y, m1, x1)});
 }
 
-// -1.0f and 0 from kernel arguments (Ne > 0): values the compiler cannot fold
+// -1.0f from a kernel argument (Ne > 0): a value the compiler cannot fold
 __device__ __forceinline__ float runtime_m1(const Geom& g) { return g.Ne > 0 ? -1.f : 1.f; }
-__device__ __forceinline__ uint32_t runtime_z0(const Geom& g) { return g.Ne > 0 ? 0u : 1u; }
 
 template <int NKS>
 __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W, const half8* __restrict__ ph16,
@@ -1076,7 +1079,6 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
   // their (0 - c) sq terms meet a zero phase and only enter the p = 0 sum,
   // which the epilogue corrects with V3P in place of V3.
   const float m1 = runtime_m1(a.g);
-  const uint32_t z0 = runtime_z0(a.g);
   auto stage2 = [&](int b, int c0, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -1094,8 +1096,8 @@ __device__ __forceinline__ void contract_mixed_body(const float2* __restrict__ W
         zsr += sr[h];
         zsi += si[h];
       }
-      split16x2(sr[0], sr[1], m1, z0, qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], m1, z0, qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
@@ -1537,7 +1539,6 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
   // first site (8 g4) of the chunk, row 4 cw, column l16: the site (+QSITE
   // float2) and row (+16) offsets are immediates.
   const float m1 = runtime_m1(a.g);
-  const uint32_t z0 = runtime_z0(a.g);
   auto stage2 = [&](int b, const float2* qb, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -1550,8 +1551,8 @@ __device__ __forceinline__ void contract_mixed_ws_body(const float2* __restrict_
         sr[h] = fmaf(q.x, sq, csr);
         si[h] = fmaf(q.y, sq, csi);
       }
-      split16x2(sr[0], sr[1], m1, z0, qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], m1, z0, qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});

@@ -291,7 +291,6 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
   // stage 2 of one chunk for row b; qb points at the real part of this
   // lane's first site (8 g4) of the chunk, row NBC cw + b, column l16
   const float m1 = runtime_m1(a.g);
-  const uint32_t z0 = runtime_z0(a.g);
   auto stage2 = [&](int b, const float* qb, float sq, int ex) {
     const float csr = -cr[b] * sq, csi = -ci[b] * sq;
     uint32_t qrh[4], qrl[4], qih[4], qil[4];
@@ -304,8 +303,8 @@ __device__ __forceinline__ void contract_mixed16_body(const uint2* __restrict__ 
         sr[h] = fmaf(q[0], sq, csr);
         si[h] = fmaf(q[QIM], sq, csi);
       }
-      split16x2(sr[0], sr[1], m1, z0, qrh[v2], qrl[v2]);
-      split16x2(si[0], si[1], m1, z0, qih[v2], qil[v2]);
+      split16x2(sr[0], sr[1], m1, qrh[v2], qrl[v2]);
+      split16x2(si[0], si[1], m1, qih[v2], qil[v2]);
     }
     const half8 Qrh = __builtin_bit_cast(half8, u4{qrh[0], qrh[1], qrh[2], qrh[3]});
     const half8 Qrl = __builtin_bit_cast(half8, u4{qrl[0], qrl[1], qrl[2], qrl[3]});
