@@ -852,7 +852,9 @@ void enqueue_stencil(ed_context* c, ed::c_t<T>* W, int tb0, int nb, hipStream_t 
       // links re-read per eigenvector chunk cost more than the plane reuse saves
       const bool lds = c->hop_lds > 0 || (c->hop_lds < 0 && sizeof(ed::c_t<T>) == 16);
       if (lds && c->Lx % ed::HL_T == 0 && c->Ly % ed::HL_T == 0 && c->Lz >= 3 && c->NeP % EC == 0) {
-        dim3 grid((c->Lx / ed::HL_T) * (c->Ly / ed::HL_T), nparent * (c->NeP / EC), nb);
+        // eigenvector chunks that hold an eigenvector (a chunk of padding only
+        // writes nothing: padded rows stay zero)
+        dim3 grid((c->Lx / ed::HL_T) * (c->Ly / ed::HL_T), nparent * ((c->Ne + EC - 1) / EC), nb);
         constexpr int NW = EC == 16 ? 16 : 8;  // one site per wave and iteration
         ed::hop3_lds_kernel<T, NW><<<grid, 64 * NW, 0, st>>>(W, c->U64.p, g, tb0, parent0, child0, nparent);
       } else {

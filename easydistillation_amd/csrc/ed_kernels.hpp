@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void hop3_kernel(c_t<T>* __restrict__ W, const
 // (the arithmetic of hop3_kernel, in its order); wave w of NW takes the
 // plane's sites SPI w ... SPI (w + 1) - 1, then SPI (w + NW) ...
 // Requires Lx, Ly multiples of 8, Lz >= 3, NeP a multiple of hl_ec<T>().
-// grid (Lx/8 * Ly/8, nparent * NeP/hl_ec<T>(), TB), block 64 * NW
+// grid (Lx/8 * Ly/8, nparent * ceil(Ne/hl_ec<T>()), TB), block 64 * NW
 // --------------------------------------------------------------------------
 constexpr int HL_T = 8, HL_H = HL_T + 2, HL_SITES = HL_H * HL_H;
 constexpr int HL_PSI_PIECES = HL_SITES * 3 * 8;      // 16-byte pieces of one plane (2400)
